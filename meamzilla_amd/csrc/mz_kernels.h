@@ -1,0 +1,86 @@
+// Kernel argument blocks and launchers of libmeamz (host-visible).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "mz_common.h"
+
+namespace mz {
+
+// Static spline layout in device memory (read by k_refresh).
+struct RefreshMeta {
+  int nspl;
+  const int* nknots;
+  const int* basis;
+  const int* koff;      // knot-vector offset of each spline
+  const double* x0;
+  const double* xn;
+  const int* bc;
+  const int* coef_off;  // first interval record of each spline
+};
+
+struct AtomArgs {
+  PotDesc pot;
+  const Chunk* chunks;
+  const int* atom_poff;   // [natoms+1] pair offsets
+  const uint2* pairs;     // [npairs] (neighbour, packed image)
+  const double* pos;      // [natoms*3]
+  const double* box;      // [ncells*9]
+  const int* types;       // [natoms]
+  const double* weight;   // [ncells]
+  const char* tables;     // [ncand][table_bytes]
+  int table_bytes;
+  int natoms;
+  int ncells;
+  long long npairs;
+  double* forces;         // [ncand][natoms*3]
+  double* cell_acc;       // [ncand][ncells][8]: E, S(6), constraint
+  double* rho_out;        // optional [ncand][natoms] density before clamping
+  double* dF_out;         // optional [ncand][natoms] U'(rho)
+  double* G_out;          // optional [ncand][npairs*3] dF-free three-body leg vectors
+};
+
+struct ResidArgs {
+  int natoms, ncells, nerr;
+  const int* cell_of;
+  const int* natoms_c;
+  const int* abase;
+  const double* weight;
+  const double* eweight;
+  const double* sweight;
+  const double* vol;
+  const double* energy0;
+  const double* stress0;
+  const double* force0;
+  double energy_weight, stress_weight;
+  const double* forces;
+  const double* cell_acc;
+  double* errvec;         // optional [ncand][nerr]
+  double* cell_energy;    // optional [ncand][ncells]
+  double* cell_stress;    // optional [ncand][ncells*6]
+  double* partial;        // [ncand][nblocks]
+};
+
+struct FinalArgs {
+  int npartial;
+  const double* partial;
+  int meam, quintic;
+  int ncells, natoms, nerr, nt, nspl, nknot;
+  int f0, nf, u0;
+  const int* koff;
+  const int* nknots;
+  const double* knots;
+  const char* tables;
+  int table_bytes;
+  double* errvec;
+  double* F;              // [ncand]
+};
+
+void launch_refresh(hipStream_t st, const double* knots, int nknot, const RefreshMeta& meta, char* tables,
+                    int table_bytes, int ncand);
+void launch_zero(hipStream_t st, double* p, size_t n);
+int launch_atoms(hipStream_t st, const AtomArgs& a, int nchunks, int ncand, int block, int ppl);
+void launch_resid(hipStream_t st, const ResidArgs& a, int nblocks, int ncand);
+void launch_final(hipStream_t st, const FinalArgs& a, int ncand);
+
+}  // namespace mz

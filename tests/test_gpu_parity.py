@@ -1,0 +1,74 @@
+"""HIP objective against the reference's numbers and the oracle (1e-10 relative,
+FP64; summation order is the only permitted difference)."""
+import numpy as np
+import pytest
+
+from conftest import golden_cases, load_case, rel_err
+
+import pyoracle
+from meamzilla_amd import gen
+from meamzilla_amd._lib import Objective
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-10
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+@pytest.mark.parametrize("case", golden_cases())
+def test_gpu_matches_reference_golden(case):
+    _say("golden", case)
+    pot, db, exp = load_case(case)
+    obj = Objective(pot, db)
+    r = obj.compute()
+    assert abs(r["F"] - exp["F"]) <= TOL * abs(exp["F"])
+    assert rel_err(r["errvec"], exp["errvec"]) < TOL
+    assert rel_err(r["cell_energy"], exp["cell_energy"]) < TOL
+    assert rel_err(r["cell_stress"], exp["cell_stress"]) < TOL
+    assert rel_err(r["forces"], exp["forces"]) < TOL
+    assert rel_err(obj.densities(), exp["densities"]) < TOL
+
+
+@pytest.mark.parametrize("cfg,scale", [(0, 1.0), (1, 24 / 256), (2, 40 / 512), (3, 30 / 1000)])
+def test_gpu_matches_oracle_workloads(cfg, scale):
+    _say("workload", cfg, scale)
+    w = gen.workload(cfg, scale=scale)
+    gen.add_targets(w, pyoracle.evaluator)
+    o = pyoracle.Oracle(w.pot, w.db)
+    obj = Objective(w.pot, w.db)
+    npairs, ntrip = o.counts()
+    assert obj.info.npairs == npairs and obj.info.ntriplets == ntrip
+    e = o.eval()
+    r = obj.compute()
+    assert abs(r["F"] - e["F"]) <= TOL * abs(e["F"])
+    for k in ("errvec", "cell_energy", "cell_stress", "forces"):
+        assert rel_err(r[k], e[k]) < TOL, k
+
+
+def test_gpu_batch_matches_plain():
+    _say("batch vs plain")
+    w = gen.workload(2, scale=20 / 512)
+    gen.add_targets(w, pyoracle.evaluator)
+    obj = Objective(w.pot, w.db)
+    K = w.candidate_knots(5)
+    Fb = obj.batch(K)
+    Fp = np.array([obj.objective(k) for k in K])
+    assert np.max(np.abs(Fb - Fp) / np.abs(Fp)) < 1e-12
+    o = pyoracle.Oracle(w.pot, w.db)
+    Fo = np.array([o.objective(k) for k in K])
+    assert np.max(np.abs(Fb - Fo) / np.abs(Fo)) < TOL
+
+
+@pytest.mark.parametrize("tile", [(16, 256, 256), (64, 512, 256), (32, 256, 128), (8, 512, 512), (64, 512, 128)])
+def test_gpu_tiles_agree(tile):
+    _say("tile", tile)
+    w = gen.workload(1, scale=6 / 256)
+    gen.add_targets(w, pyoracle.evaluator)
+    obj = Objective(w.pot, w.db)
+    obj.set_tile(*tile)
+    r = obj.compute()
+    e = pyoracle.Oracle(w.pot, w.db).eval()
+    assert abs(r["F"] - e["F"]) <= TOL * abs(e["F"])
+    assert rel_err(r["forces"], e["forces"]) < TOL
