@@ -121,6 +121,13 @@ struct mz_ctx {
   // pinned staging
   double* h_stage = nullptr;
   size_t h_stage_n = 0;
+  // captured launch sequences (forward objective of ncand vectors)
+  struct Graph {
+    hipGraphExec_t ex = nullptr;
+    int ncand = 0;
+  };
+  std::vector<Graph> graphs;
+  double* h_F = nullptr;  // pinned result buffer [ncap]
   // timing
   bool timing = false;
   hipEvent_t ev[8] = {};
@@ -372,8 +379,15 @@ int make_chunks(mz_ctx* c) {
   return MZ_OK;
 }
 
+void drop_graphs(mz_ctx* c) {
+  for (auto& g : c->graphs)
+    if (g.ex) (void)hipGraphExecDestroy(g.ex);
+  c->graphs.clear();
+}
+
 int ensure_eval_buffers(mz_ctx* c, int ncand) {
   if (ncand <= c->ncap) return MZ_OK;
+  drop_graphs(c);
   const size_t nc = (size_t)ncand;
   c->resid_blocks = (c->natoms + c->ncells + 255) / 256;
   if (c->d_knots.alloc(nc * c->nknot) || c->d_tables.alloc(nc * c->table_bytes) ||
@@ -389,6 +403,10 @@ int ensure_eval_buffers(mz_ctx* c, int ncand) {
       return fail(MZ_ERR_NOMEM, "pinned staging allocation failed");
     c->h_stage_n = need;
   }
+  if (c->h_F) (void)hipHostFree(c->h_F);
+  c->h_F = nullptr;
+  if (hipHostMalloc((void**)&c->h_F, nc * sizeof(double), hipHostMallocDefault) != hipSuccess)
+    return fail(MZ_ERR_NOMEM, "pinned result allocation failed");
   return MZ_OK;
 }
 
@@ -527,6 +545,45 @@ void mz_record_timing(mz_ctx* c) {
   c->last.atom_ms = t23;
   c->last.resid_ms = t34;
   c->last.total_ms = t04;
+}
+
+// Forward objective of ncand knot vectors through a captured graph:
+// H2D copy of the staged knots, refresh, clears, atom kernel, residual and
+// final reduction, D2H copy of F.  One graph per candidate count; graphs are
+// dropped whenever buffers or chunks change.
+int mz_forward_graph(mz_ctx* c, int ncand, const double* knots, double* F) {
+  const size_t n = (size_t)ncand * c->nknot;
+  for (size_t q = 0; q < n; ++q)
+    if (!std::isfinite(knots[q])) return fail(MZ_ERR_ARG, "knot vector holds a non-finite value");
+  std::memcpy(c->h_stage, knots, n * sizeof(double));
+  hipGraphExec_t ex = nullptr;
+  for (auto& g : c->graphs)
+    if (g.ncand == ncand) ex = g.ex;
+  if (!ex) {
+    hipGraph_t graph = nullptr;
+    MZ_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    const bool timing = c->timing;
+    c->timing = false;
+    hipError_t e1 = hipMemcpyAsync(c->d_knots.p, c->h_stage, n * sizeof(double), hipMemcpyHostToDevice, c->stream);
+    int rc = mz_run_forward(c, ncand, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
+    hipError_t e2 = hipMemcpyAsync(c->h_F, c->d_F.p, ncand * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+    c->timing = timing;
+    hipError_t e3 = hipStreamEndCapture(c->stream, &graph);
+    if (rc) return rc;
+    if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess)
+      return fail(MZ_ERR_HIP, "graph capture of the forward evaluation failed");
+    hipError_t e4 = hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e4 != hipSuccess) return fail(MZ_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e4));
+    mz_ctx::Graph g;
+    g.ex = ex;
+    g.ncand = ncand;
+    c->graphs.push_back(g);
+  }
+  MZ_HIP(hipGraphLaunch(ex, c->stream));
+  MZ_HIP(hipStreamSynchronize(c->stream));
+  std::memcpy(F, c->h_F, ncand * sizeof(double));
+  return MZ_OK;
 }
 
 // Accessors used by the adjoint and batch translation units.
@@ -698,7 +755,9 @@ void mz_destroy(mz_ctx* c) {
   c->d_pairs.release();
   c->d_chunks.release();
   c->d_tables.release();
+  drop_graphs(c);
   if (c->h_stage) (void)hipHostFree(c->h_stage);
+  if (c->h_F) (void)hipHostFree(c->h_F);
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -724,6 +783,12 @@ int mz_objective(mz_ctx* c, const double* knots, double* F, double* errvec, doub
   if (!c || !knots) return fail(MZ_ERR_ARG, "null argument");
   int rc = ensure_eval_buffers(c, 1);
   if (rc) return rc;
+  if (!errvec && !cell_energy && !cell_stress && !forces && !c->timing) {
+    double Fh = 0.0;
+    rc = mz_forward_graph(c, 1, knots, &Fh);
+    if (!rc && F) *F = Fh;
+    return rc;
+  }
   if (errvec && c->d_errvec.alloc(c->nerr)) return fail(MZ_ERR_NOMEM, "errvec allocation failed");
   if (cell_energy && c->d_cell_e.alloc(c->ncells)) return fail(MZ_ERR_NOMEM, "allocation failed");
   if (cell_stress && c->d_cell_s.alloc(6 * (size_t)c->ncells)) return fail(MZ_ERR_NOMEM, "allocation failed");
@@ -770,6 +835,7 @@ int mz_set_tile(mz_ctx* c, const mz_tile* t) {
                   (t->block == 512 && ppl == 1) || (t->block == 64 && ppl == 4);
   if (!ok || ppl * t->block != t->max_pairs) return fail(MZ_ERR_ARG, "unsupported tile (block, max_pairs)");
   mz_tile old = c->tile;
+  drop_graphs(c);
   c->tile = *t;
   if (c->tile.cand_tile < 1) c->tile.cand_tile = 1;
   int rc = make_chunks(c);
@@ -811,6 +877,7 @@ int mz_batch_impl(mz_ctx* c, int ncand, const double* knots, double* F, double* 
   if (G) return mz_grad_impl(c, ncand, knots, F, G);
   int rc = ensure_eval_buffers(c, ncand);
   if (rc) return rc;
+  if (!c->timing) return mz_forward_graph(c, ncand, knots, F);
   rc = mz_stage_knots(c, ncand, knots);
   if (rc) return rc;
   rc = mz_run_forward(c, ncand, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);

@@ -429,33 +429,58 @@ __global__ void __launch_bounds__(BLOCK) k_atoms(const AtomArgs args) {
       const int kb = sm.poff[ia[s]], ke = sm.poff[ia[s] + 1];
       const int ti = args.types[ch.a0 + ia[s]];
       const SplHead& Hg = HD[args.pot.fn_off[4] + ti];
+      // g header in registers for the whole sweep
+      const double gx0 = Hg.x0, ginv = Hg.invstep, gxcut = Hg.xcut, gxn = Hg.xn;
+      const double gkmax = (double)(Hg.n - 2);
+      const double gy0 = Hg.y0, gyn = Hg.yn, ggL = Hg.gL, ggR = Hg.gR, gg2L = Hg.g2L, gg2R = Hg.g2R;
       const SplInterval* IVg = IV + Hg.coef_off;
-      const double fj = fv[s], dfj = dfv[s], ivj = invr[s];
+      const double fj = fv[s], dfj = dfv[s];
+      const double ivg = invr[s] * ginv;  // invr_j folded with the t -> cos scale
+      const double ivj = invr[s];
       const double xj = ux[s], yj = uy[s], zj = uz[s];
       double rho_t = 0.0, gx = 0.0, gy = 0.0, gz = 0.0;
-      // Legs beyond their f cutoff carry f = f' = 0, so the triplets the
-      // point style drops (src/pots/pot_meam/triplet_meam.cpp:64-65) add
-      // exact zeros here.
+      // Every k of atom i is visited; the own pair and (knot style) partners
+      // with cos beyond x_n (1 + 1e-8) get f_k = 0, which turns their terms
+      // into exact zeros.  Legs beyond their f cutoff already carry f = f' = 0,
+      // so the triplets the point style drops (src/pots/pot_meam/triplet_meam.cpp:64-65)
+      // add zeros as well.
+#pragma unroll 2
       for (int k = kb; k < ke; ++k) {
-        if (k == p) continue;
         const double4 dk = sm.kd[k];
         const double cs = xj * dk.x + yj * dk.y + zj * dk.z;
-        double gv, dg;
-        if (POINT) {
-          double d2;
-          gv = eval_point<Q>(Hg, IV, cs, dg, d2);
+        double gv, dgt;  // g and dg/dt (t = shift within the interval)
+        bool keep = k != p;
+        if (POINT && (cs < gx0 || cs > gxn)) {
+          // linear (cubic) or quadratic (quintic) extrapolation of g
+          const bool left = cs < gx0;
+          const double e = cs - (left ? gx0 : gxn);
+          const double g1 = left ? ggL : ggR, g2 = left ? gg2L : gg2R;
+          gv = (left ? gy0 : gyn) + g1 * e + 0.5 * g2 * e * e;
+          dgt = (g1 + g2 * e) / ginv;
         } else {
-          if (!eval_knot<Q>(Hg, IVg, cs, gv, dg)) continue;
+          const double u = (cs - gx0) * ginv;
+          const double fl = fmin(fmax(floor(u), 0.0), gkmax);
+          const double t = u - fl;
+          const SplInterval& I = IVg[(int)fl];
+          if (Q) {
+            gv = I.c[0] + t * (I.c[1] + t * (I.c[2] + t * (I.c[3] + t * (I.c[4] + t * I.c[5]))));
+            dgt = I.c[1] + t * (2.0 * I.c[2] + t * (3.0 * I.c[3] + t * (4.0 * I.c[4] + t * 5.0 * I.c[5])));
+          } else {
+            gv = I.c[0] + t * (I.c[1] + t * (I.c[2] + t * I.c[3]));
+            dgt = I.c[1] + t * (2.0 * I.c[2] + t * 3.0 * I.c[3]);
+          }
+          if (!POINT) keep = keep && (cs < gxcut);
         }
-        const double fk = dk.w;
+        const double fk = keep ? dk.w : 0.0;
         const double ff = fj * fk;
         rho_t += ff * gv;
-        const double vl = dg * ff * ivj;          // V3 / dF * invr_j
-        const double vv = gv * dfj * fk - vl * cs;  // (dV3j - vlj cos) / dF
+        const double vl = dgt * ff * ivg;            // V3 / dF * invr_j
+        const double vv = gv * dfj * fk - vl * cs;   // (dV3j - vlj cos) / dF
         gx += xj * vv + dk.x * vl;
         gy += yj * vv + dk.y * vl;
         gz += zj * vv + dk.z * vl;
       }
+      (void)ivj;
       Gx[s] = gx;
       Gy[s] = gy;
       Gz[s] = gz;

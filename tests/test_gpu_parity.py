@@ -72,3 +72,21 @@ def test_gpu_tiles_agree(tile):
     e = pyoracle.Oracle(w.pot, w.db).eval()
     assert abs(r["F"] - e["F"]) <= TOL * abs(e["F"])
     assert rel_err(r["forces"], e["forces"]) < TOL
+
+
+def test_gpu_graph_replays_match_eager():
+    """The captured forward graph, replayed with eager evaluations and host
+    synchronisation in between, agrees with the eager launch sequence."""
+    _say("graph replay")
+    w = gen.workload(1, scale=8 / 256)
+    gen.add_targets(w, pyoracle.evaluator)
+    obj = Objective(w.pot, w.db)
+    K = w.candidate_knots(3)
+    for rep in range(4):
+        for k in K:
+            eager = obj.compute(k)["F"]          # outputs requested: eager path
+            graph = obj.objective(k)             # no outputs: graph replay
+            assert abs(graph - eager) <= 1e-12 * abs(eager), (rep, graph, eager)
+        Fb = obj.batch(K)                        # batched graph of 3 vectors
+        Fe = np.array([obj.compute(k)["F"] for k in K])
+        assert np.max(np.abs(Fb - Fe) / np.abs(Fe)) < 1e-12
