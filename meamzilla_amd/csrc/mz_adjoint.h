@@ -1,23 +1,9 @@
-// Reverse-sweep (adjoint) state of libmeamz.
+// Reverse-sweep (adjoint) entry points of libmeamz shared by the host units.
 #pragma once
 
-#include <hip/hip_runtime.h>
+struct mz_ctx;
 
-#include "mz_kernels.h"
-
-namespace mz {
-
-// Device buffers of the gradient path, sized for `cap` candidates.
-struct AdjointState {
-  int cap = 0;
-  double* rho = nullptr;     // [cap][natoms] unclamped densities
-  double* dF = nullptr;      // [cap][natoms] U'(rho)
-  double* G = nullptr;       // [cap][npairs*3] three-body leg vectors
-  double* ctab = nullptr;    // [cap][ncoef*8] adjoint of the interval records
-  double* head = nullptr;    // [cap][nspl*8] adjoint of the end-point data
-  double* grad = nullptr;    // [cap][nknot] gradient w.r.t. the knot vector
-};
-
-void adjoint_release(AdjointState& s);
-
-}  // namespace mz
+// objective + gradient of ncand knot vectors (F[ncand], G[ncand*nknot])
+int mz_grad_impl(mz_ctx* c, int ncand, const double* knots, double* F, double* G);
+// releases the gradient buffers and captured gradient graphs of a context
+void mz_adjoint_drop(mz_ctx* c);

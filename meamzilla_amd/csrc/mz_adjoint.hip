@@ -1,27 +1,810 @@
 // Reverse sweep: gradient of the objective with respect to the knot vector.
+//
+// With the residuals of the forward pass, the objective's sensitivities are
+//   lambda_a = dF/dF_a = 2 w (F_a - F0_a)       (forces)
+//   mu_c     = dF/dS_c = 2 e_S W_S / V          (raw stress sums)
+//   lamE_c   = dF/dE_c = 2 e_E W_E              (raw energy sums)
+//   kappa_c  = dF/dP_c = 2 P_c                  (density-range penalty)
+// Every pair p = (i -> j) carries a total force vector T_p that enters
+// F_i (+), F_j (-) and the virial -d_p (x) T_p r_p, so the force and stress
+// terms of dF reduce to sum_p w_p . dT_p with the fixed dual vector
+//   w_p = lambda_i - lambda_j - r_p v_c(d_p),  v = (mu_xx dx + mu_zx dz,
+//         mu_yy dy + mu_xy dx, mu_zz dz + mu_yz dy).
+// T_p = (phi'/2 + rho' U'_i) d_p + U'_i G_p, G_p the three-body leg sum kept by
+// the forward pass.  Per atom, S_i = sum_p (a_p rho'_p + w_p . G_p) (a_p = w_p . d_p)
+// multiplies dU'_i, and psi_i = dF/drho_i collects lamE U' + S U'' and the
+// penalty.  One sweep over the ordered triplets (j,k) of each atom then
+// accumulates, per own pair j, the adjoints of f_j and f'_j and, per g
+// interval, the adjoints of the interval's power-basis coefficients; a last
+// kernel maps coefficient adjoints through the (linear) spline refresh back to
+// the knot vector.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cmath>
+#include <cstring>
 #include <string>
+#include <vector>
 
-#include "../../include/meamz.h"
-#include "mz_adjoint.h"
+#include "mz_ctx.h"
+#include "mz_device.h"
 
 namespace mz {
 
-void adjoint_release(AdjointState& s) {
-  double** bufs[] = {&s.rho, &s.dF, &s.G, &s.ctab, &s.head, &s.grad};
-  for (double** b : bufs) {
-    if (*b) (void)hipFree(*b);
-    *b = nullptr;
+constexpr int kAdjBlock = 256;   // threads (= pairs) per adjoint workgroup
+constexpr int kAdjAtoms = 64;    // atoms per adjoint chunk
+constexpr int kGReps = 4;        // LDS replicas of the g-interval adjoints
+constexpr int kSlots = 64;       // global partial tables per candidate
+
+struct AdjArgs {
+  PotDesc pot;
+  const Chunk* chunks;
+  const int* atom_poff;
+  const uint2* pairs;
+  const double* pos;
+  const double* box;
+  const int* types;
+  const double* weight;
+  const char* tables;
+  int table_bytes;
+  int natoms, ncells;
+  long long npairs;
+  const double* rho;       // [cand][natoms] unclamped densities
+  const double* G;         // [cand][npairs*3]
+  const double* lam;       // [cand][natoms*3]
+  const double* cell_adj;  // [cand][ncells*8]
+  double* slots;           // [cand][kSlots][ntab]
+  int ntab;                // ncoef*6 + nspl*6
+  int gco0, ngco;          // first g interval record, number of g interval records
+};
+
+struct ChainArgs {
+  int nspl, ncoef, ntab, nknot, meam, nt;
+  int f0, nf, u0;
+  const int* nknots;
+  const int* basis;
+  const int* koff;
+  const int* coef_off;
+  const double* x0;
+  const double* xn;
+  const double* J;         // per spline [2n][n+4]: d(ypp, yp4)/d(knot record)
+  const int* joff;
+  const double* slots;
+  const double* knots;
+  const char* tables;
+  int table_bytes;
+  double* grad;            // [cand][nknot]
+};
+
+struct Partner {
+  double4 a;  // unit vector, f
+  double4 b;  // a f', invr f w (3)
+  double2 c;  // invr f a, unused
+};
+
+// Interval-coefficient adjoint: weight al on the value, bt on d(value)/dt.
+template <bool Q>
+__device__ __forceinline__ void acc_interval(double* tab, int rec, double t, double al, double bt) {
+  double* c = tab + rec * 6;
+  const double t2 = t * t, t3 = t2 * t;
+  atomicAdd(c + 0, al);
+  atomicAdd(c + 1, al * t + bt);
+  atomicAdd(c + 2, al * t2 + 2.0 * bt * t);
+  atomicAdd(c + 3, al * t3 + 3.0 * bt * t2);
+  if (Q) {
+    const double t4 = t3 * t;
+    atomicAdd(c + 4, al * t4 + 4.0 * bt * t3);
+    atomicAdd(c + 5, al * t4 * t + 5.0 * bt * t4);
   }
-  s.cap = 0;
+}
+
+// Adjoint of a point evaluation (value weight al, d/dr weight be) including
+// the end extrapolation; head slots per spline: y0, yn, gL, gR, g2L, g2R.
+template <bool Q>
+__device__ __forceinline__ void acc_point(double* tab, int ncoef, int s, const SplHead& H, double r, double al,
+                                          double be) {
+  double* hd = tab + ncoef * 6 + s * 6;
+  if (r < H.x0) {
+    const double e = r - H.x0;
+    atomicAdd(hd + 0, al);
+    atomicAdd(hd + 2, al * e + be);
+    if (H.basis == 5) atomicAdd(hd + 4, 0.5 * al * e * e + be * e);
+    return;
+  }
+  if (r > H.xn) {
+    const double e = r - H.xn;
+    atomicAdd(hd + 1, al);
+    atomicAdd(hd + 3, al * e + be);
+    if (H.basis == 5) atomicAdd(hd + 5, 0.5 * al * e * e + be * e);
+    return;
+  }
+  const double delta = r - H.x0;
+  int k = (int)__dmul_rn(delta, H.invstep);
+  if (k >= H.n - 1) k = H.n - 2;
+  if (k < 0) k = 0;
+  const double t = __dmul_rn(__dadd_rn(delta, -__dmul_rn((double)k, H.step)), H.invstep);
+  acc_interval<Q>(tab, H.coef_off + k, t, al, be * H.invstep);
+}
+
+// Knot-style location (same rule as eval_knot); false beyond x_n (1 + 1e-8).
+__device__ __forceinline__ bool locate_knot(const SplHead& H, const SplInterval* IVs, double r, int& k, double& t) {
+  if (!(r < H.xcut)) return false;
+  const int nm2 = H.n - 2;
+  k = (int)floor((r - H.x0) * H.invstep);
+  k = k < 0 ? 0 : (k > nm2 ? nm2 : k);
+  while (k > 0 && IVs[k].xk > r) --k;
+  while (k < nm2 && IVs[k + 1].xk <= r) ++k;
+  t = (r - IVs[k].xk) * IVs[k].invh;
+  return true;
+}
+
+// Adjoint of one radial function of a pair (value weight al, d/dr weight be).
+template <bool POINT, bool Q>
+__device__ __forceinline__ void acc_radial(double* tab, int ncoef, int s, const SplHead& H, const SplInterval* IV,
+                                           double r, double al, double be) {
+  if (POINT) {
+    if (!(r <= H.xn)) return;
+    acc_point<Q>(tab, ncoef, s, H, r, al, be);
+    return;
+  }
+  int k;
+  double t;
+  if (!locate_knot(H, IV + H.coef_off, r, k, t)) return;
+  acc_interval<Q>(tab, H.coef_off + k, t, al, be * H.invstep);
+}
+
+template <bool MEAM, bool POINT, bool Q>
+__global__ void __launch_bounds__(kAdjBlock) k_adjoint(const AdjArgs A) {
+  extern __shared__ __attribute__((aligned(64))) char dyn[];
+  __shared__ Partner pd[kAdjBlock];
+  __shared__ double S[kAdjAtoms], psi[kAdjAtoms], Up[kAdjAtoms];
+  __shared__ int poff[kAdjAtoms + 1];
+  const int cand = blockIdx.y;
+  const int tid = threadIdx.x;
+  const Chunk ch = A.chunks[blockIdx.x];
+  const int ncoef = A.pot.ncoef;
+  {
+    const int4* src = reinterpret_cast<const int4*>(A.tables + (size_t)cand * A.table_bytes);
+    int4* dst = reinterpret_cast<int4*>(dyn);
+    for (int q = tid; q < A.table_bytes / 16; q += kAdjBlock) dst[q] = src[q];
+  }
+  const SplHead* HD = reinterpret_cast<const SplHead*>(dyn);
+  const SplInterval* IV = reinterpret_cast<const SplInterval*>(dyn + sizeof(SplHead) * A.pot.nspl);
+  double* atab = reinterpret_cast<double*>(dyn + A.table_bytes);
+  double* grep = atab + A.ntab;
+  const int ngrep = MEAM ? kGReps * A.ngco * 6 : 0;
+  for (int q = tid; q < A.ntab + ngrep; q += kAdjBlock) atab[q] = 0.0;
+  for (int a = tid; a <= ch.na; a += kAdjBlock) poff[a] = A.atom_poff[ch.a0 + a] - ch.p0;
+  for (int a = tid; a < ch.na; a += kAdjBlock) S[a] = 0.0;
+  __syncthreads();
+
+  const int nt = A.pot.nt;
+  const double* box = A.box + 9 * ch.cell;
+  const double* lam = A.lam + (size_t)cand * A.natoms * 3;
+  const double* cadj = A.cell_adj + ((size_t)cand * A.ncells + ch.cell) * 8;
+  const double lamE = cadj[0];
+  const double mxx = cadj[1], myy = cadj[2], mzz = cadj[3], mxy = cadj[4], myz = cadj[5], mzx = cadj[6];
+  const double kappa = cadj[7];
+  const double wcell = A.weight[ch.cell];
+
+  // ---- phase A: pair duals, S_i, partner data
+  const int p = tid;
+  const bool has = p < ch.np;
+  int ia = 0, ti = 0, pij = 0, tj = 0;
+  double r = 0, invr = 0, ux = 0, uy = 0, uz = 0, fj = 0, dfj = 0, drho = 0, aj = 0, wx = 0, wy = 0, wz = 0;
+  if (has) {
+    int lo = 0, hi = ch.na - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (poff[mid] <= p) lo = mid; else hi = mid - 1;
+    }
+    ia = lo;
+    const int ig = ch.a0 + lo;
+    const uint2 pr = A.pairs[ch.p0 + p];
+    const int j = (int)pr.x;
+    const int ix = (int)(int8_t)(pr.y & 0xff), iy = (int)(int8_t)((pr.y >> 8) & 0xff),
+              iz = (int)(int8_t)((pr.y >> 16) & 0xff);
+    const double* pj = A.pos + 3 * (size_t)j;
+    const double* pi = A.pos + 3 * (size_t)ig;
+    double d[3];
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      double t = __dadd_rn(pj[m], __dmul_rn((double)ix, box[m]));
+      t = __dadd_rn(t, __dmul_rn((double)iy, box[3 + m]));
+      t = __dadd_rn(t, __dmul_rn((double)iz, box[6 + m]));
+      d[m] = __dadd_rn(t, -pi[m]);
+    }
+    r = __dsqrt_rn(__dadd_rn(__dadd_rn(__dmul_rn(d[0], d[0]), __dmul_rn(d[1], d[1])), __dmul_rn(d[2], d[2])));
+    invr = 1.0 / r;
+    ux = d[0] / r;
+    uy = d[1] / r;
+    uz = d[2] / r;
+    ti = A.types[ig];
+    tj = A.types[j];
+    pij = pair_ij(ti, tj, nt);
+    double v, dv;
+    if (eval_radial<POINT, Q>(HD[A.pot.fn_off[1] + tj], IV, r, v, dv)) drho = dv;
+    if (MEAM && eval_radial<POINT, Q>(HD[A.pot.fn_off[3] + pij], IV, r, v, dv)) {
+      fj = v;
+      dfj = dv;
+    }
+    const double vx = mxx * ux + mzx * uz, vy = myy * uy + mxy * ux, vz = mzz * uz + myz * uy;
+    wx = lam[3 * (size_t)ig] - lam[3 * (size_t)j] - r * vx;
+    wy = lam[3 * (size_t)ig + 1] - lam[3 * (size_t)j + 1] - r * vy;
+    wz = lam[3 * (size_t)ig + 2] - lam[3 * (size_t)j + 2] - r * vz;
+    aj = wx * ux + wy * uy + wz * uz;
+    double sv = aj * drho;
+    if (MEAM) {
+      const double* G = A.G + ((size_t)cand * A.npairs + ch.p0 + p) * 3;
+      sv += wx * G[0] + wy * G[1] + wz * G[2];
+      Partner P;
+      P.a = make_double4(ux, uy, uz, fj);
+      P.b = make_double4(aj * dfj, invr * fj * wx, invr * fj * wy, invr * fj * wz);
+      P.c = make_double2(invr * fj * aj, 0.0);
+      pd[p] = P;
+    }
+    atomicAdd(&S[ia], sv);
+  }
+  __syncthreads();
+
+  // ---- phase B: per-atom density sensitivity and the embedding adjoint
+  for (int a = tid; a < ch.na; a += kAdjBlock) {
+    const double rho = A.rho[(size_t)cand * A.natoms + ch.a0 + a];
+    const int su = A.pot.fn_off[2] + A.types[ch.a0 + a];
+    const SplHead& HU = HD[su];
+    const bool below = rho < HU.x0, above = rho > HU.xn;
+    const bool out = below || above;
+    const double bound = below ? HU.x0 : HU.xn;
+    const double rt = (out && !A.pot.embed_extrap) ? bound : rho;
+    double dU, d2U;
+    (void)eval_point<Q>(HU, IV, rt, dU, d2U);
+    double ps = 0.0;
+    if (!out || A.pot.embed_extrap) ps = lamE * dU + S[a] * d2U;
+    if (out) ps += kappa * wcell * kDummyWeight * 10. * 2.0 * (rho - bound);
+    psi[a] = ps;
+    Up[a] = dU;
+    acc_point<Q>(atab, ncoef, su, HU, rt, lamE, S[a]);
+  }
+  __syncthreads();
+
+  // ---- phase C: ordered triplets (j = own pair, k = partner)
+  double Phi = 0.0, Phip = 0.0;
+  if (MEAM && has) {
+    const double ps = psi[ia], up = Up[ia];
+    const int kb = poff[ia], ke = poff[ia + 1];
+    const int sg = A.pot.fn_off[4] + ti;
+    const SplHead& Hg = HD[sg];
+    const double gx0 = Hg.x0, ginv = Hg.invstep, gxcut = Hg.xcut, gxn = Hg.xn;
+    const double gkmax = (double)(Hg.n - 2);
+    const double gy0 = Hg.y0, gyn = Hg.yn, ggL = Hg.gL, ggR = Hg.gR, gg2L = Hg.g2L, gg2R = Hg.g2R;
+    const bool gq = Hg.basis == 5;
+    const SplInterval* IVg = IV + Hg.coef_off;
+    double* gtab = grep + (tid & (kGReps - 1)) * A.ngco * 6;
+    const int gbase = Hg.coef_off - A.gco0;
+    double* ghead = atab + ncoef * 6 + sg * 6;
+    for (int k = kb; k < ke; ++k) {
+      const Partner P = pd[k];
+      const double cs = ux * P.a.x + uy * P.a.y + uz * P.a.z;
+      double gv, dgt, t = 0.0, fl = 0.0;
+      bool extrap = false, keep = k != p;
+      if (POINT && (cs < gx0 || cs > gxn)) {
+        extrap = true;
+        const bool left = cs < gx0;
+        const double e = cs - (left ? gx0 : gxn);
+        const double g1 = left ? ggL : ggR, g2 = left ? gg2L : gg2R;
+        gv = (left ? gy0 : gyn) + g1 * e + 0.5 * g2 * e * e;
+        dgt = (g1 + g2 * e) / ginv;
+        t = e;
+        fl = left ? 0.0 : 1.0;
+      } else {
+        const double u = (cs - gx0) * ginv;
+        fl = fmin(fmax(floor(u), 0.0), gkmax);
+        t = u - fl;
+        const SplInterval& I = IVg[(int)fl];
+        if (Q) {
+          gv = I.c[0] + t * (I.c[1] + t * (I.c[2] + t * (I.c[3] + t * (I.c[4] + t * I.c[5]))));
+          dgt = I.c[1] + t * (2.0 * I.c[2] + t * (3.0 * I.c[3] + t * (4.0 * I.c[4] + t * 5.0 * I.c[5])));
+        } else {
+          gv = I.c[0] + t * (I.c[1] + t * (I.c[2] + t * I.c[3]));
+          dgt = I.c[1] + t * (2.0 * I.c[2] + t * 3.0 * I.c[3]);
+        }
+        if (!POINT) keep = keep && (cs < gxcut);
+      }
+      const double gp = dgt * ginv;
+      const double fk = keep ? P.a.w : 0.0;
+      const double Ak = keep ? P.b.x : 0.0;
+      const double Wkx = keep ? P.b.y : 0.0, Wky = keep ? P.b.z : 0.0, Wkz = keep ? P.b.w : 0.0;
+      const double Ck = keep ? P.c.x : 0.0;
+      const double B = invr * (wx * P.a.x + wy * P.a.y + wz * P.a.z - aj * cs);
+      const double ff = fj * fk;
+      const double al = 0.5 * ps * ff + up * aj * dfj * fk;
+      const double be = up * B * ff;
+      Phi += ps * fk * gv + up * (B * gp * fk + gv * Ak + gp * (Wkx * ux + Wky * uy + Wkz * uz - Ck * cs));
+      Phip += up * aj * gv * fk;
+      if (POINT && extrap) {
+        const bool left = fl == 0.0;
+        const double e = t;
+        atomicAdd(ghead + (left ? 0 : 1), al);
+        atomicAdd(ghead + (left ? 2 : 3), al * e + be);
+        if (gq) atomicAdd(ghead + (left ? 4 : 5), 0.5 * al * e * e + be * e);
+      } else {
+        acc_interval<Q>(gtab, gbase + (int)fl, t, al, be * ginv);
+      }
+    }
+  }
+
+  // ---- phase D: pair functions
+  if (has) {
+    const double ps = psi[ia], up = Up[ia];
+    const int sphi = A.pot.fn_off[0] + pij, srho = A.pot.fn_off[1] + tj;
+    acc_radial<POINT, Q>(atab, ncoef, sphi, HD[sphi], IV, r, 0.5 * lamE, 0.5 * aj);
+    acc_radial<POINT, Q>(atab, ncoef, srho, HD[srho], IV, r, ps, aj * up);
+    if (MEAM) {
+      const int sf = A.pot.fn_off[3] + pij;
+      acc_radial<POINT, Q>(atab, ncoef, sf, HD[sf], IV, r, Phi, Phip);
+    }
+  }
+  __syncthreads();
+  if (MEAM) {
+    for (int q = tid; q < A.ngco * 6; q += kAdjBlock) {
+      double sacc = 0.0;
+#pragma unroll
+      for (int rep = 0; rep < kGReps; ++rep) sacc += grep[rep * A.ngco * 6 + q];
+      atab[A.gco0 * 6 + q] += sacc;
+    }
+    __syncthreads();
+  }
+  double* slot = A.slots + ((size_t)cand * kSlots + (blockIdx.x % kSlots)) * A.ntab;
+  for (int q = tid; q < A.ntab; q += kAdjBlock) {
+    const double v = atab[q];
+    if (v != 0.0) unsafeAtomicAdd(slot + q, v);
+  }
+}
+
+// One 64-lane workgroup per (spline, candidate): slot reduction, penalties,
+// interval coefficients -> (y, y'', y'''') -> knot record.
+__global__ void __launch_bounds__(64) k_chain(const ChainArgs A) {
+  const int s = blockIdx.x, cand = blockIdx.y, lane = threadIdx.x;
+  const int n = A.nknots[s], ni = n - 1, off = A.coef_off[s];
+  __shared__ double cb[(kMaxKnots - 1) * 6], hb[6], yb[kMaxKnots], db[kMaxKnots], qb[kMaxKnots],
+      rb[kMaxKnots + 4];
+  const double* slots = A.slots + (size_t)cand * kSlots * A.ntab;
+  for (int q = lane; q < ni * 6 + 6; q += 64) {
+    const int idx = q < ni * 6 ? off * 6 + q : A.ncoef * 6 + s * 6 + (q - ni * 6);
+    double sacc = 0.0;
+    for (int sl = 0; sl < kSlots; ++sl) sacc += slots[(size_t)sl * A.ntab + idx];
+    if (q < ni * 6) cb[q] = sacc; else hb[q - ni * 6] = sacc;
+  }
+  __syncthreads();
+  if (lane == 0) {
+    const char* tb = A.tables + (size_t)cand * A.table_bytes;
+    const SplHead* HD = reinterpret_cast<const SplHead*>(tb);
+    const SplInterval* IV = reinterpret_cast<const SplInterval*>(tb + sizeof(SplHead) * A.nspl);
+    const SplHead& H = HD[s];
+    const double* kv = A.knots + (size_t)cand * A.nknot;
+    if (!A.meam && s >= A.u0 && s < A.u0 + A.nt) {
+      // U'(midpoint) penalty, src/pots/pot_eam_spline/pot_eam_spline.cpp:211-218
+      const double mid = 0.5 * (H.x0 + H.xn);
+      double d, d2;
+      if (H.basis == 5) eval_point<true>(H, IV, mid, d, d2);
+      else eval_point<false>(H, IV, mid, d, d2);
+      const double be = 2.0 * (kDummyWeight * d) * kDummyWeight;
+      const double delta = mid - H.x0;
+      int k = (int)__dmul_rn(delta, H.invstep);
+      if (k >= H.n - 1) k = H.n - 2;
+      if (k < 0) k = 0;
+      const double t = __dmul_rn(__dadd_rn(delta, -__dmul_rn((double)k, H.step)), H.invstep);
+      const double bt = be * H.invstep;
+      double tp = 1.0;
+      for (int m = 1; m < 6; ++m) {
+        cb[k * 6 + m] += m * bt * tp;
+        tp *= t;
+      }
+    }
+    const double h = H.step;
+    const double H2 = h * h / 6.0, H4 = (h * h * h * h) / 360.0;
+    for (int k = 0; k < n; ++k) yb[k] = db[k] = qb[k] = 0.0;
+    for (int k = 0; k < ni; ++k) {
+      const double* c = cb + 6 * k;
+      yb[k] += c[0] - c[1];
+      yb[k + 1] += c[1];
+      db[k] += H2 * (-2.0 * c[1] + 3.0 * c[2] - c[3]);
+      db[k + 1] += H2 * (-c[1] + c[3]);
+      qb[k] += H4 * (8.0 * c[1] - 20.0 * c[3] + 15.0 * c[4] - 3.0 * c[5]);
+      qb[k + 1] += H4 * (7.0 * c[1] - 10.0 * c[3] + 3.0 * c[5]);
+    }
+    yb[0] += hb[0];
+    yb[n - 1] += hb[1];
+    for (int k = 0; k < n; ++k) rb[k] = yb[k];
+    rb[n] = hb[2];
+    rb[n + 1] = hb[3];
+    rb[n + 2] = A.basis[s] == 5 ? hb[4] : 0.0;
+    rb[n + 3] = A.basis[s] == 5 ? hb[5] : 0.0;
+    if (A.meam && s >= A.f0 && s < A.f0 + A.nf) {
+      // f-max penalty (src/pots/pot_meam_spline/pot_meam_spline.cpp:274-278):
+      // only the knot holding the signed max |y| over all f splines moves it
+      double mx = 0.0;
+      int ms = -1, mk = -1;
+      for (int q = 0; q < A.nf; ++q) {
+        const int s2 = A.f0 + q;
+        const double* y = kv + A.koff[s2];
+        double m = 0.0;
+        int km = -1;
+        for (int k = 0; k < A.nknots[s2]; ++k)
+          if (fabs(m) < fabs(y[k])) {
+            m = y[k];
+            km = k;
+          }
+        if (fabs(mx) < fabs(m)) {
+          mx = m;
+          ms = s2;
+          mk = km;
+        }
+      }
+      if (ms == s && mk >= 0) {
+        const double m = fabs(mx);
+        const double pe = kDummyWeight * 25. * (1.0 - m) * (1.0 - m);
+        rb[mk] += -2.0 * pe * kDummyWeight * 25. * 2.0 * (1.0 - m) * (mx > 0 ? 1.0 : -1.0);
+      }
+    }
+  }
+  __syncthreads();
+  const double* J = A.J + A.joff[s];
+  for (int rr = lane; rr < n + 4; rr += 64) {
+    double v = rb[rr];
+    for (int q = 0; q < n; ++q) v += J[(size_t)q * (n + 4) + rr] * db[q];
+    if (A.basis[s] == 5)
+      for (int q = 0; q < n; ++q) v += J[(size_t)(n + q) * (n + 4) + rr] * qb[q];
+    A.grad[(size_t)cand * A.nknot + A.koff[s] + rr] = v;
+  }
 }
 
 }  // namespace mz
 
+// ------------------------------------------------------------------------ host
+
+namespace {
+
+// y'' (and y'''' for quintic) of one spline from its knot record; the same
+// algorithms as the refresh kernel (Thomas sweep / Gaussian elimination).
+void host_refresh(int basis, int n, double left, double right, int natL, int natR, const double* rec, double* ypp,
+                  double* yp4) {
+  const double step = (right - left) / (n - 1), invstep = 1. / step;
+  const double* y = rec;
+  const double gL = rec[n], gR = rec[n + 1], g2L = rec[n + 2], g2R = rec[n + 3];
+  if (basis == 3) {
+    std::vector<double> u(n, 0.0);
+    if (natL) {
+      ypp[0] = 0;
+    } else {
+      ypp[0] = -0.5;
+      u[0] = (3.0 * invstep) * ((y[1] - y[0]) * invstep - gL);
+    }
+    for (int i = 1; i < n - 1; ++i) {
+      const double p = 0.5 * ypp[i - 1] + 2.0;
+      ypp[i] = (-0.5) / p;
+      u[i] = (y[i + 1] - y[i]) * invstep - (y[i] - y[i - 1]) * invstep;
+      u[i] = (6.0 * u[i] * 0.5 * invstep - 0.5 * u[i - 1]) / p;
+    }
+    double qn = 0.0, un = 0.0;
+    if (!natR) {
+      qn = 0.5;
+      un = (3.0 * invstep) * (gR - (y[n - 1] - y[n - 2]) * invstep);
+    }
+    ypp[n - 1] = (un - qn * u[n - 2]) / (qn * ypp[n - 2] + 1.0);
+    for (int k = n - 2; k >= 0; --k) ypp[k] = ypp[k] * ypp[k + 1] + u[k];
+    for (int k = 0; k < n; ++k) yp4[k] = 0.0;
+    return;
+  }
+  const int nd = 2 * n;
+  std::vector<double> A((size_t)nd * nd, 0.0), b(nd, 0.0);
+  auto a = [&](int r, int c) -> double& { return A[(size_t)r * nd + c]; };
+  for (int i = 1; i < n - 1; ++i) {
+    a(i, i - 1) = 1. / 12; a(i, i) = 10. / 12; a(i, i + 1) = 1. / 12;
+    a(i, n + i - 1) = 1. / 180; a(i, n + i) = -1. / 90; a(i, n + i + 1) = 1. / 180;
+    a(n + i, i - 1) = 1.; a(n + i, i) = -2.; a(n + i, i + 1) = 1.;
+    a(n + i, n + i - 1) = 1. / 6; a(n + i, n + i) = 2. / 3; a(n + i, n + i + 1) = 1. / 6;
+    b[i] = y[i + 1] - 2 * y[i] + y[i - 1];
+  }
+  if (natL) {
+    a(0, 0) = -1.; a(0, 1) = 1.; a(0, n) = 1. / 3; a(0, n + 1) = 1. / 6;
+    a(n, n) = 1.;
+  } else {
+    a(0, 0) = 1. / 3; a(0, 1) = 1. / 6; a(0, n) = 1. / 45; a(0, n + 1) = 7. / 360;
+    b[0] = y[1] - y[0] - gL * step;
+    a(n, 0) = 1.; b[n] = g2L * (step * step);
+  }
+  if (natR) {
+    a(n - 1, n - 2) = -1.; a(n - 1, n - 1) = 1.; a(n - 1, 2 * n - 2) = -1. / 6; a(n - 1, 2 * n - 1) = -1. / 3;
+    a(2 * n - 1, 2 * n - 1) = 1.;
+  } else {
+    a(n - 1, n - 2) = -1. / 6; a(n - 1, n - 1) = -1. / 3; a(n - 1, 2 * n - 2) = -7. / 360; a(n - 1, 2 * n - 1) = -1. / 45;
+    b[n - 1] = y[n - 1] - y[n - 2] - gR * step;
+    a(2 * n - 1, n - 1) = 1.; b[2 * n - 1] = g2R * (step * step);
+  }
+  for (int k = 0; k < nd; ++k) {
+    int p = k;
+    double m = std::fabs(a(k, k));
+    for (int i = k + 1; i < nd; ++i)
+      if (std::fabs(a(i, k)) > m) { m = std::fabs(a(i, k)); p = i; }
+    if (p != k) {
+      for (int c = 0; c < nd; ++c) std::swap(a(k, c), a(p, c));
+      std::swap(b[k], b[p]);
+    }
+    for (int i = k + 1; i < nd; ++i) {
+      const double l = a(i, k) / a(k, k);
+      if (l == 0.0) continue;
+      a(i, k) = 0.0;
+      for (int c = k + 1; c < nd; ++c) a(i, c) -= l * a(k, c);
+      b[i] -= l * b[k];
+    }
+  }
+  for (int k = nd - 1; k >= 0; --k) {
+    double sacc = b[k];
+    for (int c = k + 1; c < nd; ++c) sacc -= a(k, c) * b[c];
+    b[k] = sacc / a(k, k);
+  }
+  for (int i = 0; i < n; ++i) {
+    ypp[i] = b[i] * (invstep * invstep);
+    yp4[i] = -b[n + i] * (invstep * invstep * invstep * invstep);
+  }
+}
+
+#define ADJ_HIP(call)                                                                 \
+  do {                                                                               \
+    hipError_t e_ = (call);                                                          \
+    if (e_ != hipSuccess) return mz_fail(MZ_ERR_HIP, std::string(#call ": ") + hipGetErrorString(e_)); \
+  } while (0)
+
+int setup_adjoint(mz_ctx* c) {
+  mz::AdjointExtra& X = c->adjx;
+  if (X.ready) return MZ_OK;
+  // refresh Jacobians (linear maps knot record -> y'', y'''')
+  std::vector<double> J;
+  std::vector<int> joff(c->nspl);
+  for (int s = 0; s < c->nspl; ++s) {
+    const int n = c->nknots[s];
+    joff[s] = (int)J.size();
+    std::vector<double> col(2 * (size_t)n * (n + 4), 0.0);
+    std::vector<double> rec(n + 4), ypp(n), yp4(n);
+    for (int r = 0; r < n + 4; ++r) {
+      std::fill(rec.begin(), rec.end(), 0.0);
+      rec[r] = 1.0;
+      host_refresh(c->basis[s], n, c->x0[s], c->xn[s], c->bc[2 * s] == -1, c->bc[2 * s + 1] == -1, rec.data(),
+                   ypp.data(), yp4.data());
+      for (int q = 0; q < n; ++q) {
+        col[(size_t)q * (n + 4) + r] = ypp[q];
+        col[(size_t)(n + q) * (n + 4) + r] = yp4[q];
+      }
+    }
+    J.insert(J.end(), col.begin(), col.end());
+  }
+  if (X.d_J.upload(J.data(), J.size()) || X.d_joff.upload(joff.data(), joff.size()))
+    return mz_fail(MZ_ERR_NOMEM, "adjoint Jacobian upload failed");
+  // chunks of at most 256 pairs for the adjoint workgroups
+  std::vector<mz::Chunk> ch;
+  for (int cc = 0; cc < c->ncells; ++cc) {
+    int a = c->abase[cc];
+    const int aend = a + c->natoms_c[cc];
+    while (a < aend) {
+      mz::Chunk k{};
+      k.cell = cc;
+      k.a0 = a;
+      k.p0 = c->atom_poff[a];
+      int na = 0;
+      while (a < aend && na < mz::kAdjAtoms) {
+        const int np_a = c->atom_poff[a + 1] - c->atom_poff[a];
+        if (np_a > mz::kAdjBlock)
+          return mz_fail(MZ_ERR_CAPACITY, "atom " + std::to_string(a) + " has more than 256 neighbours");
+        if (c->atom_poff[a + 1] - k.p0 > mz::kAdjBlock) break;
+        ++a;
+        ++na;
+      }
+      k.na = na;
+      k.np = c->atom_poff[a] - k.p0;
+      ch.push_back(k);
+    }
+  }
+  std::vector<long long> work(ch.size());
+  for (size_t q = 0; q < ch.size(); ++q) {
+    long long w = ch[q].np;
+    for (int a = ch[q].a0; a < ch[q].a0 + ch[q].na; ++a) w += 2 * c->ntrip_atom[a];
+    work[q] = w;
+  }
+  std::vector<size_t> order(ch.size());
+  for (size_t q = 0; q < order.size(); ++q) order[q] = q;
+  std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return work[x] > work[y]; });
+  std::vector<mz::Chunk> sorted(ch.size());
+  for (size_t q = 0; q < ch.size(); ++q) sorted[q] = ch[order[q]];
+  if (X.d_chunks.upload(sorted.data(), sorted.size())) return mz_fail(MZ_ERR_NOMEM, "adjoint chunk upload failed");
+  X.nchunks = (int)sorted.size();
+  X.ntab = c->ncoef * 6 + c->nspl * 6;
+  X.gco0 = c->meam ? c->coef_off[c->pot.fn_off[4]] : 0;
+  X.ngco = c->meam ? c->ncoef - X.gco0 : 0;
+  X.dyn_bytes = c->table_bytes + (X.ntab + (c->meam ? mz::kGReps * X.ngco * 6 : 0)) * (int)sizeof(double);
+  if (X.dyn_bytes + (int)(sizeof(mz::Partner) * mz::kAdjBlock) + 4096 > 64 * 1024)
+    return mz_fail(MZ_ERR_CAPACITY, "adjoint workgroup would exceed 64 KB of LDS");
+  X.ready = true;
+  return MZ_OK;
+}
+
+int ensure_adjoint_buffers(mz_ctx* c, int ncand) {
+  mz::AdjointExtra& X = c->adjx;
+  if (ncand <= X.cap) return MZ_OK;
+  const size_t nc = (size_t)ncand;
+  bool bad = X.d_rho.alloc(nc * c->natoms) || X.d_dF.alloc(nc * c->natoms) || X.d_G.alloc(nc * c->npairs * 3) ||
+             X.d_lam.alloc(nc * c->natoms * 3) || X.d_cadj.alloc(nc * c->ncells * 8) ||
+             X.d_slots.alloc(nc * mz::kSlots * X.ntab) || X.d_grad.alloc(nc * c->nknot);
+  if (bad) return mz_fail(MZ_ERR_NOMEM, "device allocation of gradient buffers failed");
+  if (X.h_grad) (void)hipHostFree(X.h_grad);
+  X.h_grad = nullptr;
+  if (hipHostMalloc((void**)&X.h_grad, nc * c->nknot * sizeof(double), hipHostMallocDefault) != hipSuccess)
+    return mz_fail(MZ_ERR_NOMEM, "pinned gradient allocation failed");
+  X.cap = ncand;
+  for (auto& g : X.graphs)
+    if (g.ex) (void)hipGraphExecDestroy(g.ex);
+  X.graphs.clear();
+  return MZ_OK;
+}
+
+template <bool M, bool P, bool Q>
+void launch_adjoint_t(hipStream_t st, const mz::AdjArgs& a, int nchunks, int ncand, int dyn) {
+  hipLaunchKernelGGL((mz::k_adjoint<M, P, Q>), dim3(nchunks, ncand), dim3(mz::kAdjBlock), dyn, st, a);
+}
+
+}  // namespace
+
+int mz_run_forward(mz_ctx* c, int ncand, double* d_errvec, double* d_ce, double* d_cs, double* d_rho, double* d_dF,
+                   double* d_G, double* d_lam, double* d_cadj);
+mz::AtomArgs mz_atom_args(const mz_ctx* c);
+int mz_ensure_buffers(mz_ctx* c, int ncand);
+
+// Launch sequence of objective + gradient for ncand staged knot vectors.
+static int run_grad(mz_ctx* c, int ncand) {
+  mz::AdjointExtra& X = c->adjx;
+  hipStream_t st = c->stream;
+  int rc = mz_run_forward(c, ncand, nullptr, nullptr, nullptr, X.d_rho.p, X.d_dF.p, c->meam ? X.d_G.p : nullptr,
+                          X.d_lam.p, X.d_cadj.p);
+  if (rc) return rc;
+  mz::launch_zero(st, X.d_slots.p, (size_t)ncand * mz::kSlots * X.ntab);
+  if (c->timing) ADJ_HIP(hipEventRecord(c->ev[5], st));
+  mz::AdjArgs a{};
+  a.pot = c->pot;
+  a.chunks = X.d_chunks.p;
+  a.atom_poff = c->d_atom_poff.p;
+  a.pairs = c->d_pairs.p;
+  a.pos = c->d_pos.p;
+  a.box = c->d_box.p;
+  a.types = c->d_types.p;
+  a.weight = c->d_weight.p;
+  a.tables = c->d_tables.p;
+  a.table_bytes = c->table_bytes;
+  a.natoms = c->natoms;
+  a.ncells = c->ncells;
+  a.npairs = c->npairs;
+  a.rho = X.d_rho.p;
+  a.G = X.d_G.p;
+  a.lam = X.d_lam.p;
+  a.cell_adj = X.d_cadj.p;
+  a.slots = X.d_slots.p;
+  a.ntab = X.ntab;
+  a.gco0 = X.gco0;
+  a.ngco = X.ngco;
+  const bool q = c->pot.quintic != 0;
+  if (c->meam) {
+    if (c->point) { if (q) launch_adjoint_t<true, true, true>(st, a, X.nchunks, ncand, X.dyn_bytes);
+                    else launch_adjoint_t<true, true, false>(st, a, X.nchunks, ncand, X.dyn_bytes); }
+    else { if (q) launch_adjoint_t<true, false, true>(st, a, X.nchunks, ncand, X.dyn_bytes);
+           else launch_adjoint_t<true, false, false>(st, a, X.nchunks, ncand, X.dyn_bytes); }
+  } else {
+    if (c->point) { if (q) launch_adjoint_t<false, true, true>(st, a, X.nchunks, ncand, X.dyn_bytes);
+                    else launch_adjoint_t<false, true, false>(st, a, X.nchunks, ncand, X.dyn_bytes); }
+    else { if (q) launch_adjoint_t<false, false, true>(st, a, X.nchunks, ncand, X.dyn_bytes);
+           else launch_adjoint_t<false, false, false>(st, a, X.nchunks, ncand, X.dyn_bytes); }
+  }
+  ADJ_HIP(hipGetLastError());
+  if (c->timing) ADJ_HIP(hipEventRecord(c->ev[6], st));
+  mz::ChainArgs ca{};
+  ca.nspl = c->nspl;
+  ca.ncoef = c->ncoef;
+  ca.ntab = X.ntab;
+  ca.nknot = c->nknot;
+  ca.meam = c->meam;
+  ca.nt = c->nt;
+  ca.f0 = c->pot.fn_off[3];
+  ca.nf = c->nt * (c->nt + 1) / 2;
+  ca.u0 = c->pot.fn_off[2];
+  ca.nknots = c->d_nknots.p;
+  ca.basis = c->d_basis.p;
+  ca.koff = c->d_koff.p;
+  ca.coef_off = c->d_coef_off.p;
+  ca.x0 = c->d_x0.p;
+  ca.xn = c->d_xn.p;
+  ca.J = X.d_J.p;
+  ca.joff = X.d_joff.p;
+  ca.slots = X.d_slots.p;
+  ca.knots = c->d_knots.p;
+  ca.tables = c->d_tables.p;
+  ca.table_bytes = c->table_bytes;
+  ca.grad = X.d_grad.p;
+  hipLaunchKernelGGL(mz::k_chain, dim3(c->nspl, ncand), dim3(64), 0, st, ca);
+  ADJ_HIP(hipGetLastError());
+  return MZ_OK;
+}
+
 int mz_grad_impl(mz_ctx* c, int ncand, const double* knots, double* F, double* G) {
-  (void)c; (void)ncand; (void)knots; (void)F; (void)G;
-  return MZ_ERR_ARG;
+  if (!c || !knots || !F || !G || ncand < 1) return mz_fail(MZ_ERR_ARG, "null argument or ncand < 1");
+  int rc = setup_adjoint(c);
+  if (rc) return rc;
+  mz::AdjointExtra& X = c->adjx;
+  // bound the memory of the stored leg vectors: sub-batches of candidates
+  const size_t per_cand = (size_t)c->npairs * 3 * sizeof(double) + (size_t)c->natoms * 48 + (size_t)c->ncells * 64;
+  int sub = (int)std::max<size_t>(1, std::min<size_t>((size_t)ncand, (size_t)(24ull << 30) / std::max<size_t>(per_cand, 1)));
+  for (int c0 = 0; c0 < ncand; c0 += sub) {
+    const int nb = std::min(sub, ncand - c0);
+    rc = mz_ensure_buffers(c, nb);
+    if (rc) return rc;
+    rc = ensure_adjoint_buffers(c, nb);
+    if (rc) return rc;
+    const double* kv = knots + (size_t)c0 * c->nknot;
+    const size_t n = (size_t)nb * c->nknot;
+    for (size_t q = 0; q < n; ++q)
+      if (!std::isfinite(kv[q])) return mz_fail(MZ_ERR_ARG, "knot vector holds a non-finite value");
+    std::memcpy(c->h_stage, kv, n * sizeof(double));
+    hipStream_t st = c->stream;
+    if (c->timing) {
+      ADJ_HIP(hipMemcpyAsync(c->d_knots.p, c->h_stage, n * sizeof(double), hipMemcpyHostToDevice, st));
+      rc = run_grad(c, nb);
+      if (rc) return rc;
+      ADJ_HIP(hipMemcpyAsync(c->h_F, c->d_F.p, nb * sizeof(double), hipMemcpyDeviceToHost, st));
+      ADJ_HIP(hipMemcpyAsync(X.h_grad, X.d_grad.p, n * sizeof(double), hipMemcpyDeviceToHost, st));
+      ADJ_HIP(hipStreamSynchronize(st));
+      float ms = 0;
+      (void)hipEventElapsedTime(&ms, c->ev[5], c->ev[6]);
+      c->last.adjoint_ms = ms;
+    } else {
+      hipGraphExec_t ex = nullptr;
+      for (auto& g : X.graphs)
+        if (g.ncand == nb) ex = g.ex;
+      if (!ex) {
+        hipGraph_t graph = nullptr;
+        ADJ_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        hipError_t e1 = hipMemcpyAsync(c->d_knots.p, c->h_stage, n * sizeof(double), hipMemcpyHostToDevice, st);
+        int rc2 = run_grad(c, nb);
+        hipError_t e2 = hipMemcpyAsync(c->h_F, c->d_F.p, nb * sizeof(double), hipMemcpyDeviceToHost, st);
+        hipError_t e3 = hipMemcpyAsync(X.h_grad, X.d_grad.p, n * sizeof(double), hipMemcpyDeviceToHost, st);
+        hipError_t e4 = hipStreamEndCapture(st, &graph);
+        if (rc2) return rc2;
+        if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess || e4 != hipSuccess)
+          return mz_fail(MZ_ERR_HIP, "graph capture of the gradient evaluation failed");
+        hipError_t e5 = hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e5 != hipSuccess) return mz_fail(MZ_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e5));
+        mz::AdjointExtra::Graph g;
+        g.ex = ex;
+        g.ncand = nb;
+        X.graphs.push_back(g);
+      }
+      ADJ_HIP(hipGraphLaunch(ex, st));
+      ADJ_HIP(hipStreamSynchronize(st));
+    }
+    std::memcpy(F + c0, c->h_F, nb * sizeof(double));
+    std::memcpy(G + (size_t)c0 * c->nknot, X.h_grad, n * sizeof(double));
+  }
+  return MZ_OK;
+}
+
+void mz_adjoint_drop(mz_ctx* c) {
+  mz::AdjointExtra& X = c->adjx;
+  for (auto& g : X.graphs)
+    if (g.ex) (void)hipGraphExecDestroy(g.ex);
+  X.graphs.clear();
+  X.d_rho.release(); X.d_dF.release(); X.d_G.release(); X.d_lam.release(); X.d_cadj.release();
+  X.d_slots.release(); X.d_grad.release(); X.d_J.release(); X.d_joff.release(); X.d_chunks.release();
+  if (X.h_grad) (void)hipHostFree(X.h_grad);
+  X.h_grad = nullptr;
+  X.cap = 0;
+  X.ready = false;
 }
 
 extern "C" int mz_objective_grad(mz_ctx* c, const double* knots, double* F, double* dF) {

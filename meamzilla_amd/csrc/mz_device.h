@@ -1,0 +1,110 @@
+// Device helpers shared by the forward and adjoint kernels: reductions and
+// spline evaluation from the per-interval power-basis tables.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "mz_common.h"
+
+namespace mz {
+
+// ------------------------------------------------------------------ helpers
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+
+template <int BLOCK>
+__device__ __forceinline__ double block_sum(double v, double* scratch) {
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) scratch[wid] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int w = 0; w < BLOCK / 64; ++w) s += scratch[w];
+  }
+  return s;  // valid in thread 0
+}
+
+__device__ __forceinline__ int pair_ij(int ti, int tj, int nt) {
+  return (ti <= tj) ? ti * nt + tj - ((ti * (ti + 1)) >> 1) : tj * nt + ti - ((tj * (tj + 1)) >> 1);
+}
+
+template <bool Q>
+__device__ __forceinline__ void poly(const SplInterval& I, double t, double invstep, double& v, double& d) {
+  if (Q) {
+    v = I.c[0] + t * (I.c[1] + t * (I.c[2] + t * (I.c[3] + t * (I.c[4] + t * I.c[5]))));
+    d = (I.c[1] + t * (2.0 * I.c[2] + t * (3.0 * I.c[3] + t * (4.0 * I.c[4] + t * 5.0 * I.c[5])))) * invstep;
+  } else {
+    v = I.c[0] + t * (I.c[1] + t * (I.c[2] + t * I.c[3]));
+    d = (I.c[1] + t * (2.0 * I.c[2] + t * 3.0 * I.c[3])) * invstep;
+  }
+}
+
+// Knot-style evaluation (Spline::get_knotshift src/spline.cpp:221-246 then
+// splint_comb by knot).  Returns false when r lies beyond x_n (1 + 1e-8).
+template <bool Q>
+__device__ __forceinline__ bool eval_knot(const SplHead& H, const SplInterval* IV, double r, double& v, double& d) {
+  if (!(r < H.xcut)) return false;
+  const int nm2 = H.n - 2;
+  int k = (int)floor((r - H.x0) * H.invstep);
+  k = k < 0 ? 0 : (k > nm2 ? nm2 : k);
+  while (k > 0 && IV[k].xk > r) --k;
+  while (k < nm2 && IV[k + 1].xk <= r) ++k;
+  const SplInterval& I = IV[k];
+  const double t = (r - I.xk) * I.invh;
+  poly<Q>(I, t, H.invstep, v, d);
+  return true;
+}
+
+// Point evaluation with end extrapolation (splint_comb(double) and
+// splint_grad2(double), src/spline3eq.h:204-266, src/spline5eq.h:218-286).
+template <bool Q>
+__device__ __forceinline__ double eval_point(const SplHead& H, const SplInterval* IV, double r, double& d, double& d2) {
+  if (r < H.x0) {
+    const double e = r - H.x0;
+    d = H.gL + H.g2L * e;
+    d2 = H.g2L;
+    return H.y0 + H.gL * e + 0.5 * H.g2L * e * e;
+  }
+  if (r > H.xn) {
+    const double e = r - H.xn;
+    d = H.gR + H.g2R * e;
+    d2 = H.g2R;
+    return H.yn + H.gR * e + 0.5 * H.g2R * e * e;
+  }
+  const double delta = r - H.x0;
+  int k = (int)__dmul_rn(delta, H.invstep);
+  if (k >= H.n - 1) k = H.n - 2;
+  if (k < 0) k = 0;
+  const double t = __dmul_rn(__dadd_rn(delta, -__dmul_rn((double)k, H.step)), H.invstep);
+  const SplInterval& I = IV[H.coef_off + k];
+  double v;
+  poly<Q>(I, t, H.invstep, v, d);
+  if (Q)
+    d2 = (2.0 * I.c[2] + t * (6.0 * I.c[3] + t * (12.0 * I.c[4] + t * 20.0 * I.c[5]))) * H.invstep2;
+  else
+    d2 = (2.0 * I.c[2] + t * 6.0 * I.c[3]) * H.invstep2;
+  return v;
+}
+
+// Evaluates one radial function of a pair: knot style (spline styles) or point
+// style ("eam"/"meam": inside iff r <= x_n).  Returns false when outside.
+template <bool POINT, bool Q>
+__device__ __forceinline__ bool eval_radial(const SplHead& H, const SplInterval* IV, double r, double& v,
+                                            double& d) {
+  if (POINT) {
+    if (!(r <= H.xn)) return false;
+    double d2;
+    v = eval_point<Q>(H, IV, r, d, d2);
+    return true;
+  }
+  return eval_knot<Q>(H, IV + H.coef_off, r, v, d);
+}
+
+}  // namespace mz

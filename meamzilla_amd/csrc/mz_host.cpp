@@ -15,6 +15,10 @@
 #include "mz_common.h"
 #include "mz_kernels.h"
 #include "mz_adjoint.h"
+#include "mz_ctx.h"
+
+using mz::DevBuf;
+using mz::Grid;
 
 namespace {
 
@@ -32,107 +36,11 @@ int fail(int code, const std::string& msg) {
       return fail(MZ_ERR_HIP, std::string(#call ": ") + hipGetErrorString(e_));             \
   } while (0)
 
-template <class T>
-struct DevBuf {
-  T* p = nullptr;
-  size_t n = 0;
-  int alloc(size_t count) {
-    if (count <= n && p) return 0;
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    n = 0;
-    if (count == 0) count = 1;
-    if (hipMalloc((void**)&p, count * sizeof(T)) != hipSuccess) return -1;
-    n = count;
-    return 0;
-  }
-  int upload(const T* h, size_t count) {
-    if (alloc(count)) return -1;
-    if (count && hipMemcpy(p, h, count * sizeof(T), hipMemcpyHostToDevice) != hipSuccess) return -1;
-    return 0;
-  }
-  void release() {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    n = 0;
-  }
-};
-
 int pair_ij_h(int ti, int tj, int nt) {
   return (ti <= tj) ? ti * nt + tj - ((ti * (ti + 1)) / 2) : tj * nt + ti - ((tj * (tj + 1)) / 2);
 }
 
-// Host copy of one spline's knot grid (Spline3Eq::set_spline src/spline3eq.cpp:354-370).
-struct Grid {
-  int n;
-  double x0, xn, step;
-  std::vector<double> x;
-  void init(int n_, double l, double r) {
-    n = n_;
-    x0 = l;
-    xn = r;
-    step = (r - l) / (n - 1);
-    x.assign(n, 0.0);
-    x[0] = l;
-    x[n - 1] = r;
-    for (int i = 1; i < n - 1; ++i) x[i] = l + step * i;
-  }
-  double xcut() const { return x[n - 1] * (1. + 1.e-8); }
-};
-
 }  // namespace
-
-struct mz_ctx {
-  mz::PotDesc pot{};
-  int style = 0, nt = 0, nspl = 0;
-  bool meam = false, point = false;
-  std::vector<int> basis, nknots, bc, koff, coef_off;
-  std::vector<double> x0, xn;
-  std::vector<Grid> grids;
-  int nknot = 0, ncoef = 0, table_bytes = 0;
-  // database
-  int ncells = 0, natoms = 0, nerr = 0;
-  std::vector<int> natoms_c, abase, cell_of, types;
-  std::vector<double> vol;
-  // lists
-  std::vector<int> atom_poff;
-  std::vector<uint2> pairs;
-  std::vector<long long> ntrip_atom;
-  long long npairs = 0, ntriplets = 0;
-  std::vector<mz::Chunk> chunks;
-  mz_tile tile{64, 512, 256, 1};
-  // device: static
-  hipStream_t stream = nullptr;
-  int device = 0;
-  DevBuf<mz::Chunk> d_chunks;
-  DevBuf<int> d_atom_poff, d_types, d_cell_of, d_natoms_c, d_abase;
-  DevBuf<uint2> d_pairs;
-  DevBuf<double> d_pos, d_box, d_weight, d_eweight, d_sweight, d_vol, d_energy0, d_stress0, d_force0;
-  DevBuf<int> d_nknots, d_basis, d_koff, d_bc, d_coef_off;
-  DevBuf<double> d_x0, d_xn;
-  double energy_weight = 0, stress_weight = 0;
-  // device: per evaluation (sized for ncap candidates)
-  int ncap = 0;
-  DevBuf<double> d_knots, d_forces, d_cell_acc, d_partial, d_F, d_errvec, d_cell_e, d_cell_s, d_rho, d_dF, d_G;
-  DevBuf<char> d_tables;
-  int resid_blocks = 0;
-  // adjoint state
-  mz::AdjointState adj;
-  // pinned staging
-  double* h_stage = nullptr;
-  size_t h_stage_n = 0;
-  // captured launch sequences (forward objective of ncand vectors)
-  struct Graph {
-    hipGraphExec_t ex = nullptr;
-    int ncand = 0;
-  };
-  std::vector<Graph> graphs;
-  double* h_F = nullptr;  // pinned result buffer [ncap]
-  // timing
-  bool timing = false;
-  hipEvent_t ev[8] = {};
-  mz_timing last{};
-};
 
 namespace {
 
@@ -383,6 +291,9 @@ void drop_graphs(mz_ctx* c) {
   for (auto& g : c->graphs)
     if (g.ex) (void)hipGraphExecDestroy(g.ex);
   c->graphs.clear();
+  for (auto& g : c->adjx.graphs)
+    if (g.ex) (void)hipGraphExecDestroy(g.ex);
+  c->adjx.graphs.clear();
 }
 
 int ensure_eval_buffers(mz_ctx* c, int ncand) {
@@ -495,7 +406,7 @@ mz::FinalArgs final_args(const mz_ctx* c) {
 // Launch sequence of one batch of ncand evaluations whose knot vectors are
 // already in d_knots.  Shared by the plain, gradient and batched entry points.
 int mz_run_forward(mz_ctx* c, int ncand, double* d_errvec, double* d_ce, double* d_cs, double* d_rho, double* d_dF,
-                   double* d_G) {
+                   double* d_G, double* d_lam, double* d_cadj) {
   hipStream_t st = c->stream;
   if (c->timing) MZ_HIP(hipEventRecord(c->ev[0], st));
   mz::launch_refresh(st, c->d_knots.p, c->nknot, refresh_meta(c), c->d_tables.p, c->table_bytes, ncand);
@@ -516,6 +427,8 @@ int mz_run_forward(mz_ctx* c, int ncand, double* d_errvec, double* d_ce, double*
   ra.errvec = d_errvec;
   ra.cell_energy = d_ce;
   ra.cell_stress = d_cs;
+  ra.lam = d_lam;
+  ra.cell_adj = d_cadj;
   mz::launch_resid(st, ra, c->resid_blocks, ncand);
   mz::FinalArgs fa = final_args(c);
   fa.errvec = d_errvec;
@@ -565,7 +478,7 @@ int mz_forward_graph(mz_ctx* c, int ncand, const double* knots, double* F) {
     const bool timing = c->timing;
     c->timing = false;
     hipError_t e1 = hipMemcpyAsync(c->d_knots.p, c->h_stage, n * sizeof(double), hipMemcpyHostToDevice, c->stream);
-    int rc = mz_run_forward(c, ncand, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
+    int rc = mz_run_forward(c, ncand, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
     hipError_t e2 = hipMemcpyAsync(c->h_F, c->d_F.p, ncand * sizeof(double), hipMemcpyDeviceToHost, c->stream);
     c->timing = timing;
     hipError_t e3 = hipStreamEndCapture(c->stream, &graph);
@@ -588,7 +501,6 @@ int mz_forward_graph(mz_ctx* c, int ncand, const double* knots, double* F) {
 
 // Accessors used by the adjoint and batch translation units.
 mz::AtomArgs mz_atom_args(const mz_ctx* c) { return atom_args(c); }
-mz::AdjointState& mz_adjoint_state(mz_ctx* c) { return c->adj; }
 
 extern "C" {
 
@@ -743,7 +655,7 @@ int mz_create(const mz_layout* layout, const mz_db* db, mz_ctx** out) {
 void mz_destroy(mz_ctx* c) {
   if (!c) return;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  mz::adjoint_release(c->adj);
+  mz_adjoint_drop(c);
   DevBuf<double>* dbufs[] = {&c->d_pos, &c->d_box, &c->d_weight, &c->d_eweight, &c->d_sweight, &c->d_vol,
                              &c->d_energy0, &c->d_stress0, &c->d_force0, &c->d_x0, &c->d_xn, &c->d_knots,
                              &c->d_forces, &c->d_cell_acc, &c->d_partial, &c->d_F, &c->d_errvec, &c->d_cell_e,
@@ -795,7 +707,7 @@ int mz_objective(mz_ctx* c, const double* knots, double* F, double* errvec, doub
   rc = mz_stage_knots(c, 1, knots);
   if (rc) return rc;
   rc = mz_run_forward(c, 1, errvec ? c->d_errvec.p : nullptr, cell_energy ? c->d_cell_e.p : nullptr,
-                      cell_stress ? c->d_cell_s.p : nullptr, nullptr, nullptr, nullptr);
+                      cell_stress ? c->d_cell_s.p : nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
   if (rc) return rc;
   hipStream_t st = c->stream;
   double Fh = 0.0;
@@ -820,7 +732,7 @@ int mz_densities(mz_ctx* c, const double* knots, double* dens) {
   if (c->d_rho.alloc(c->natoms)) return fail(MZ_ERR_NOMEM, "allocation failed");
   rc = mz_stage_knots(c, 1, knots);
   if (rc) return rc;
-  rc = mz_run_forward(c, 1, nullptr, nullptr, nullptr, c->d_rho.p, nullptr, nullptr);
+  rc = mz_run_forward(c, 1, nullptr, nullptr, nullptr, c->d_rho.p, nullptr, nullptr, nullptr, nullptr);
   if (rc) return rc;
   MZ_HIP(hipMemcpyAsync(dens, c->d_rho.p, sizeof(double) * c->natoms, hipMemcpyDeviceToHost, c->stream));
   MZ_HIP(hipStreamSynchronize(c->stream));
@@ -870,7 +782,6 @@ int mz_ensure_buffers(mz_ctx* c, int ncand) { return ensure_eval_buffers(c, ncan
 
 // Batched scoring of ncand knot vectors in one launch sequence (forward only
 // when G is NULL; the gradient path lives in mz_adjoint.hip).
-int mz_grad_impl(mz_ctx* c, int ncand, const double* knots, double* F, double* G);
 
 int mz_batch_impl(mz_ctx* c, int ncand, const double* knots, double* F, double* G) {
   if (!c || !knots || !F || ncand < 1) return fail(MZ_ERR_ARG, "null argument or ncand < 1");
@@ -880,10 +791,12 @@ int mz_batch_impl(mz_ctx* c, int ncand, const double* knots, double* F, double* 
   if (!c->timing) return mz_forward_graph(c, ncand, knots, F);
   rc = mz_stage_knots(c, ncand, knots);
   if (rc) return rc;
-  rc = mz_run_forward(c, ncand, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
+  rc = mz_run_forward(c, ncand, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
   if (rc) return rc;
   MZ_HIP(hipMemcpyAsync(F, c->d_F.p, sizeof(double) * ncand, hipMemcpyDeviceToHost, c->stream));
   MZ_HIP(hipStreamSynchronize(c->stream));
   mz_record_timing(c);
   return MZ_OK;
 }
+
+int mz_fail(int code, const std::string& msg) { return fail(code, msg); }

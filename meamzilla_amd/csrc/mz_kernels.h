@@ -59,6 +59,8 @@ struct ResidArgs {
   double* cell_energy;    // optional [ncand][ncells]
   double* cell_stress;    // optional [ncand][ncells*6]
   double* partial;        // [ncand][nblocks]
+  double* lam;            // optional [ncand][natoms*3]: dF/d(force) = 2 w (F - F0)
+  double* cell_adj;       // optional [ncand][ncells*8]: dF/dE_raw, dF/dS_raw (6), dF/dP (constraint)
 };
 
 struct FinalArgs {

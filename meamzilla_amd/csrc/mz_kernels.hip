@@ -23,107 +23,9 @@
 
 #include "mz_common.h"
 #include "mz_kernels.h"
+#include "mz_device.h"
 
 namespace mz {
-
-// ------------------------------------------------------------------ helpers
-
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-  return v;
-}
-
-template <int BLOCK>
-__device__ __forceinline__ double block_sum(double v, double* scratch) {
-  v = wave_sum(v);
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  __syncthreads();
-  if (lane == 0) scratch[wid] = v;
-  __syncthreads();
-  double s = 0.0;
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int w = 0; w < BLOCK / 64; ++w) s += scratch[w];
-  }
-  return s;  // valid in thread 0
-}
-
-__device__ __forceinline__ int pair_ij(int ti, int tj, int nt) {
-  return (ti <= tj) ? ti * nt + tj - ((ti * (ti + 1)) >> 1) : tj * nt + ti - ((tj * (tj + 1)) >> 1);
-}
-
-template <bool Q>
-__device__ __forceinline__ void poly(const SplInterval& I, double t, double invstep, double& v, double& d) {
-  if (Q) {
-    v = I.c[0] + t * (I.c[1] + t * (I.c[2] + t * (I.c[3] + t * (I.c[4] + t * I.c[5]))));
-    d = (I.c[1] + t * (2.0 * I.c[2] + t * (3.0 * I.c[3] + t * (4.0 * I.c[4] + t * 5.0 * I.c[5])))) * invstep;
-  } else {
-    v = I.c[0] + t * (I.c[1] + t * (I.c[2] + t * I.c[3]));
-    d = (I.c[1] + t * (2.0 * I.c[2] + t * 3.0 * I.c[3])) * invstep;
-  }
-}
-
-// Knot-style evaluation (Spline::get_knotshift src/spline.cpp:221-246 then
-// splint_comb by knot).  Returns false when r lies beyond x_n (1 + 1e-8).
-template <bool Q>
-__device__ __forceinline__ bool eval_knot(const SplHead& H, const SplInterval* IV, double r, double& v, double& d) {
-  if (!(r < H.xcut)) return false;
-  const int nm2 = H.n - 2;
-  int k = (int)floor((r - H.x0) * H.invstep);
-  k = k < 0 ? 0 : (k > nm2 ? nm2 : k);
-  while (k > 0 && IV[k].xk > r) --k;
-  while (k < nm2 && IV[k + 1].xk <= r) ++k;
-  const SplInterval& I = IV[k];
-  const double t = (r - I.xk) * I.invh;
-  poly<Q>(I, t, H.invstep, v, d);
-  return true;
-}
-
-// Point evaluation with end extrapolation (splint_comb(double) and
-// splint_grad2(double), src/spline3eq.h:204-266, src/spline5eq.h:218-286).
-template <bool Q>
-__device__ __forceinline__ double eval_point(const SplHead& H, const SplInterval* IV, double r, double& d, double& d2) {
-  if (r < H.x0) {
-    const double e = r - H.x0;
-    d = H.gL + H.g2L * e;
-    d2 = H.g2L;
-    return H.y0 + H.gL * e + 0.5 * H.g2L * e * e;
-  }
-  if (r > H.xn) {
-    const double e = r - H.xn;
-    d = H.gR + H.g2R * e;
-    d2 = H.g2R;
-    return H.yn + H.gR * e + 0.5 * H.g2R * e * e;
-  }
-  const double delta = r - H.x0;
-  int k = (int)__dmul_rn(delta, H.invstep);
-  if (k >= H.n - 1) k = H.n - 2;
-  if (k < 0) k = 0;
-  const double t = __dmul_rn(__dadd_rn(delta, -__dmul_rn((double)k, H.step)), H.invstep);
-  const SplInterval& I = IV[H.coef_off + k];
-  double v;
-  poly<Q>(I, t, H.invstep, v, d);
-  if (Q)
-    d2 = (2.0 * I.c[2] + t * (6.0 * I.c[3] + t * (12.0 * I.c[4] + t * 20.0 * I.c[5]))) * H.invstep2;
-  else
-    d2 = (2.0 * I.c[2] + t * 6.0 * I.c[3]) * H.invstep2;
-  return v;
-}
-
-// Evaluates one radial function of a pair: knot style (spline styles) or point
-// style ("eam"/"meam": inside iff r <= x_n).  Returns false when outside.
-template <bool POINT, bool Q>
-__device__ __forceinline__ bool eval_radial(const SplHead& H, const SplInterval* IV, double r, double& v,
-                                            double& d) {
-  if (POINT) {
-    if (!(r <= H.xn)) return false;
-    double d2;
-    v = eval_point<Q>(H, IV, r, d, d2);
-    return true;
-  }
-  return eval_knot<Q>(H, IV + H.coef_off, r, v, d);
-}
 
 // ---------------------------------------------------------------- k_refresh
 
@@ -578,26 +480,33 @@ __global__ void __launch_bounds__(BLOCK) k_resid(const ResidArgs a) {
       const double e = w * (F[m] - a.force0[3 * (size_t)idx + m]);
       sq += e * e;
       if (ev) ev[e0 + m] = e;
+      if (a.lam) a.lam[((size_t)cand * a.natoms + idx) * 3 + m] = 2.0 * e * w;
     }
   } else if (idx < a.natoms + a.ncells) {
     const int c = idx - a.natoms;
     const int n = a.natoms_c[c];
     const double E = cacc[8 * c] / n;
     const size_t e0 = 7 * (size_t)c + 3 * (size_t)a.abase[c];
-    const double eE = a.weight[c] * a.eweight[c] * a.energy_weight * n * (E - a.energy0[c]);
+    const double WE = a.weight[c] * a.eweight[c] * a.energy_weight;
+    const double eE = WE * n * (E - a.energy0[c]);
     sq += eE * eE;
+    double* adj = a.cell_adj ? a.cell_adj + ((size_t)cand * a.ncells + c) * 8 : nullptr;
+    if (adj) adj[0] = 2.0 * eE * WE;
     if (ev) ev[e0] = eE;
     if (a.cell_energy) a.cell_energy[(size_t)cand * a.ncells + c] = E;
 #pragma unroll
     for (int m = 0; m < 6; ++m) {
       const double S = cacc[8 * c + 1 + m] / a.vol[c];
-      const double es = a.weight[c] * a.sweight[c] * a.stress_weight * (S - a.stress0[6 * c + m]);
+      const double WS = a.weight[c] * a.sweight[c] * a.stress_weight;
+      const double es = WS * (S - a.stress0[6 * c + m]);
       sq += es * es;
+      if (adj) adj[1 + m] = 2.0 * es * WS / a.vol[c];
       if (ev) ev[e0 + 1 + m] = es;
       if (a.cell_stress) a.cell_stress[((size_t)cand * a.ncells + c) * 6 + m] = S;
     }
     const double con = cacc[8 * c + 7];
     sq += con * con;
+    if (adj) adj[7] = 2.0 * con;
     if (ev) ev[7 * (size_t)a.ncells + 3 * (size_t)a.natoms + c] = con;
   }
   const double t = block_sum<BLOCK>(sq, red);

@@ -115,6 +115,28 @@ class PotentialSpec:
     def ncoeff(self) -> int:
         return sum(s.ncoeff() for s in self.splines)
 
+    def coeff_slots(self) -> np.ndarray:
+        """Knot-vector index of every adjustable coefficient, in the order the
+        reference's Potential::coeff(i) walks them (src/potential.cpp:192-206,
+        src/pot_fns.cpp:209-218, src/spline.cpp:252-261): per spline its free
+        knots, then its free boundary values -- one slot per free side for cubic
+        splines (src/spline3eq.cpp:376-382), slope then second derivative for
+        quintic ones (src/spline5eq.cpp:461-471).  The walk is global while each
+        quintic free side counts once in ncoeff, so the list is the first
+        ncoeff entries of the concatenated walk."""
+        walk = []
+        off = 0
+        for s in self.splines:
+            n = s.n
+            walk += [off + k for k in range(n) if not s.y_fixed[k]]
+            for side in (0, 1):
+                if s.bc[side] == 0:
+                    walk.append(off + n + side)
+                    if s.basis == 5:
+                        walk.append(off + n + 2 + side)
+            off += n + 4
+        return np.array(walk[: self.ncoeff()], dtype=np.int64)
+
     def copy(self) -> "PotentialSpec":
         return PotentialSpec(self.style, self.ntypes,
                              [dataclasses.replace(s, y=s.y.copy(), y_fixed=s.y_fixed.copy())

@@ -1,0 +1,85 @@
+"""Adjoint gradient: against central differences of the GPU objective, against
+the reference's own central-difference gradients (golden fixtures) and against
+central differences of the oracle."""
+import numpy as np
+import pytest
+
+from conftest import golden_cases, load_case
+
+import pyoracle
+from meamzilla_amd import gen
+from meamzilla_amd._lib import Objective
+
+pytestmark = pytest.mark.gpu
+GTOL = 1e-6
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+def _cd(fun, kv, idx, h_rel=1e-6):
+    g = np.zeros(len(idx))
+    for n, i in enumerate(idx):
+        h = h_rel * max(abs(kv[i]), 1.0)
+        kp = kv.copy(); kp[i] += h
+        km = kv.copy(); km[i] -= h
+        g[n] = (fun(kp) - fun(km)) / (2 * h)
+    return g
+
+
+def _rel(a, b):
+    return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
+
+
+@pytest.mark.parametrize("case", golden_cases())
+def test_adjoint_matches_gpu_central_differences(case):
+    _say("adjoint vs CD", case)
+    pot, db, exp = load_case(case)
+    obj = Objective(pot, db)
+    kv = pot.knot_vector()
+    F, g = obj.objective_grad(kv)
+    assert abs(F - exp["F"]) <= 1e-10 * abs(exp["F"])
+    idx = np.arange(len(kv))
+    gcd = _cd(obj.objective, kv, idx)
+    assert _rel(g, gcd) < GTOL, (case, _rel(g, gcd))
+
+
+@pytest.mark.parametrize("case", [c for c in golden_cases() if load_case(c)[2]["fd_grad"]])
+def test_adjoint_matches_reference_fd_gradient(case):
+    _say("adjoint vs reference FD", case)
+    pot, db, exp = load_case(case)
+    obj = Objective(pot, db)
+    F, g = obj.objective_grad()
+    gc = g[pot.coeff_slots()]
+    ref = np.array(exp["fd_grad"])
+    assert _rel(gc, ref) < GTOL, (case, _rel(gc, ref))
+
+
+@pytest.mark.parametrize("cfg,scale", [(1, 6 / 256), (2, 10 / 512)])
+def test_adjoint_matches_oracle_cd(cfg, scale):
+    _say("adjoint vs oracle CD", cfg)
+    w = gen.workload(cfg, scale=scale)
+    gen.add_targets(w, pyoracle.evaluator)
+    o = pyoracle.Oracle(w.pot, w.db)
+    obj = Objective(w.pot, w.db)
+    kv = w.pot.knot_vector()
+    F, g = obj.objective_grad(kv)
+    slots = w.pot.coeff_slots()
+    gcd = _cd(o.objective, kv, slots)
+    assert _rel(g[slots], gcd) < GTOL
+
+
+def test_batched_gradient_matches_single():
+    _say("batched gradient")
+    w = gen.workload(2, scale=12 / 512)
+    gen.add_targets(w, pyoracle.evaluator)
+    obj = Objective(w.pot, w.db)
+    K = w.candidate_knots(6)
+    Fb, Gb = obj.batch(K, grad=True)
+    for c in range(len(K)):
+        F, g = obj.objective_grad(K[c])
+        assert abs(Fb[c] - F) <= 1e-12 * abs(F)
+        assert _rel(Gb[c], g) < 1e-12
+    for c in range(len(K)):
+        assert abs(Fb[c] - obj.objective(K[c])) <= 1e-12 * abs(Fb[c])
