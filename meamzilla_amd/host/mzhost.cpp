@@ -884,7 +884,7 @@ std::vector<Potential*> genalg(const std::vector<Potential*>& pop_in, const OptO
       std::copy(kv.begin(), kv.end(), K.begin() + i * nknot);
     }
     check(mz_objective_batch(evaluator->ctx(), (int)ps.size(), K.data(), F.data(), nullptr));
-    for (size_t i = 0; i < ps.size(); ++i) ps[i]->compute();  // keeps error_sum_ and the call count in step
+    for (size_t i = 0; i < ps.size(); ++i) ps[i]->set_error(F[i]);
   };
   score(pop);
   std::stable_sort(pop.begin(), pop.end(), [](Potential* a, Potential* b) { return *a < *b; });
@@ -948,7 +948,7 @@ int run_param_file(const std::string& param_path, std::ostream& out) {
   if (slash != std::string::npos) dir = param_path.substr(0, slash + 1);
   auto path = [&](const std::string& f) { return (!f.empty() && f[0] == '/') ? f : dir + f; };
   if (!prm.has("config") || !prm.has("startpot")) throw Error("Parameter not found in file: config / startpot");
-  PotFile pf = read_potential_file(path(prm.str("config").empty() ? "" : prm.str("startpot")));
+  PotFile pf = read_potential_file(path(prm.str("startpot")));
   ConfigDb db = read_config_file(path(prm.str("config")));
   db.energy_weight = prm.num("energy_weight", 0.0);  // src/config.cpp:27-28 defaults
   db.stress_weight = prm.num("stress_weight", 0.0);

@@ -6,10 +6,12 @@
 // gradient.
 #pragma once
 
+#include <functional>
 #include <map>
 #include <memory>
 #include <ostream>
 #include <random>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -104,6 +106,7 @@ class Potential {
   void set_dir(const std::vector<double>& dir, double x);    // src/potential.cpp:268-274
   double compute_dir(ErrorVec* ev, const std::vector<double>& dir, double x);  // :281-292
   double get_error() const { return error_sum_; }
+  void set_error(double F) { error_sum_ = F; }
   static unsigned long long get_ncalls() { return ncalls_; }
   static void reset_ncalls() { ncalls_ = 0; }
 

@@ -1,0 +1,61 @@
+"""The C++ host driver (meamz_amd): file readers and coefficient walk on the CPU,
+end-to-end fits through the C-ABI on the GPU."""
+import os
+import re
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, ROOT, golden_cases, load_case
+
+EXE = os.path.join(ROOT, "meamzilla_amd", "meamz_amd")
+
+
+@pytest.mark.parametrize("case", golden_cases())
+def test_driver_reads_files_and_walks_coefficients(case):
+    pot, db, exp = load_case(case)
+    out = subprocess.run([EXE, "--check", os.path.join(GOLDEN, case, "param")], capture_output=True, text=True,
+                         check=True).stdout
+    head, coeffs = out.strip().split("\n")
+    assert f"ncoeff {exp['ncoeff']} " in head and f"atoms {db.total_atoms}" in head
+    vals = np.array([float(v) for v in coeffs.split()[1:]])
+    assert np.array_equal(vals, np.array(exp["coeffs"]))
+
+
+def _run_fit(tmp_path, case, optstyle, extra=()):
+    d = tmp_path / case
+    shutil.copytree(os.path.join(GOLDEN, case), d)
+    with open(d / "param", "a") as fh:
+        fh.write(f"optstyle {optstyle}\n")
+        for e in extra:
+            fh.write(e + "\n")
+    print("driver", case, optstyle, flush=True)
+    res = subprocess.run([EXE, "-p", str(d / "param")], capture_output=True, text=True, timeout=600)
+    assert res.returncode == 0, res.stderr
+    ini = [float(x) for x in re.findall(r"Initial total sum error = ([0-9.eE+-]+)", res.stdout)]
+    fin = [float(x) for x in re.findall(r"Final total sum error = ([0-9.eE+-]+)", res.stdout)]
+    return ini, fin, res.stdout, d
+
+
+@pytest.mark.gpu
+def test_driver_lbfgs_decreases_objective(tmp_path):
+    pot, db, exp = load_case("meam_spline_c1")
+    ini, fin, out, d = _run_fit(tmp_path, "meam_spline_c1", "lbfgs", ["max_steps 40"])
+    assert abs(ini[0] - exp["F"]) <= 1e-6 * exp["F"]   # printed with 6 decimals
+    assert fin[0] < 0.5 * ini[0]
+    assert os.path.exists(d / "pot.end")
+
+
+@pytest.mark.gpu
+def test_driver_powell_decreases_objective(tmp_path):
+    pot, db, exp = load_case("eam_spline_c0")
+    ini, fin, out, d = _run_fit(tmp_path, "eam_spline_c0", "powell", ["max_steps 2"])
+    assert fin[0] < ini[0]
+
+
+@pytest.mark.gpu
+def test_driver_genalg_runs_batched(tmp_path):
+    ini, fin, out, d = _run_fit(tmp_path, "eam_spline_c0", "genalg", ["max_steps 3", "pop_size 8", "seed 5"])
+    assert "generation 3" in out and fin[0] <= ini[0] * 10
