@@ -105,6 +105,23 @@ def cpu_baseline(w, seconds: float) -> dict:
                 sample=f"{n} objective evaluations of the full workload by the C restatement, {cores} threads")
 
 
+def _copy_probe(mib: int) -> None:
+    """One hipMemcpy device-to-device of a known size: the copy kernel's
+    FETCH_SIZE / WRITE_SIZE counters calibrate the traffic of the other kernels."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    n = mib << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(n)) == 0 and hip.hipMalloc(C.byref(b), C.c_size_t(n)) == 0
+    hip.hipMemset(a, 1, C.c_size_t(n))
+    hip.hipDeviceSynchronize()
+    assert hip.hipMemcpy(b, a, C.c_size_t(n), 3) == 0  # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log(f"copy probe: {mib} MiB device-to-device")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -116,12 +133,16 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tile", type=str, default="", help="max_atoms,max_pairs,block")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="also run one device-to-device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (configs shard over GPUs by design "
               f"only, see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
 
+    if args.copy_probe:
+        _copy_probe(args.copy_probe)
     t0 = time.time()
     w = gen.workload(args.workload)
     gen.add_targets(w, gpu_evaluator)

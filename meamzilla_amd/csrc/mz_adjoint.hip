@@ -738,8 +738,9 @@ int mz_grad_impl(mz_ctx* c, int ncand, const double* knots, double* F, double* G
   if (rc) return rc;
   mz::AdjointExtra& X = c->adjx;
   // bound the memory of the stored leg vectors: sub-batches of candidates
-  const size_t per_cand = (size_t)c->npairs * 3 * sizeof(double) + (size_t)c->natoms * 48 + (size_t)c->ncells * 64;
-  int sub = (int)std::max<size_t>(1, std::min<size_t>((size_t)ncand, (size_t)(24ull << 30) / std::max<size_t>(per_cand, 1)));
+  const size_t per_cand = (size_t)c->npairs * 6 * sizeof(double) + (size_t)c->natoms * 120 + (size_t)c->ncells * 64 +
+                          c->chunks.size() * 64 + (size_t)mz::kSlots * X.ntab * 8;
+  int sub = (int)std::max<size_t>(1, std::min<size_t>((size_t)ncand, (size_t)(48ull << 30) / std::max<size_t>(per_cand, 1)));
   for (int c0 = 0; c0 < ncand; c0 += sub) {
     const int nb = std::min(sub, ncand - c0);
     rc = mz_ensure_buffers(c, nb);

@@ -94,6 +94,8 @@ struct mz_ctx {
   std::vector<long long> ntrip_atom;
   long long npairs = 0, ntriplets = 0;
   std::vector<mz::Chunk> chunks;
+  std::vector<int> in_off, in_list;           // CSR: pairs whose neighbour is each atom
+  std::vector<int> cell_chunk_off, cell_chunks; // CSR: chunks of every cell
   mz_tile tile{64, 512, 256, 1};
   // device: static
   hipStream_t stream = nullptr;
@@ -101,13 +103,14 @@ struct mz_ctx {
   mz::DevBuf<mz::Chunk> d_chunks;
   mz::DevBuf<int> d_atom_poff, d_types, d_cell_of, d_natoms_c, d_abase;
   mz::DevBuf<uint2> d_pairs;
+  mz::DevBuf<int> d_in_off, d_in_list, d_cell_chunk_off, d_cell_chunks;
   mz::DevBuf<double> d_pos, d_box, d_weight, d_eweight, d_sweight, d_vol, d_energy0, d_stress0, d_force0;
   mz::DevBuf<int> d_nknots, d_basis, d_koff, d_bc, d_coef_off;
   mz::DevBuf<double> d_x0, d_xn;
   double energy_weight = 0, stress_weight = 0;
   // device: per evaluation (sized for ncap candidates)
   int ncap = 0;
-  mz::DevBuf<double> d_knots, d_forces, d_cell_acc, d_partial, d_F, d_errvec, d_cell_e, d_cell_s, d_rho, d_dF, d_G;
+  mz::DevBuf<double> d_knots, d_forces, d_tvec, d_fown, d_chunk_acc, d_partial, d_F, d_errvec, d_cell_e, d_cell_s, d_rho, d_dF, d_G;
   mz::DevBuf<char> d_tables;
   int resid_blocks = 0;
   // gradient path

@@ -283,7 +283,20 @@ int make_chunks(mz_ctx* c) {
   std::vector<mz::Chunk> sorted(order.size());
   for (size_t q = 0; q < order.size(); ++q) sorted[q] = c->chunks[order[q]];
   c->chunks.swap(sorted);
-  if (c->d_chunks.upload(c->chunks.data(), c->chunks.size())) return fail(MZ_ERR_NOMEM, "chunk upload failed");
+  c->cell_chunk_off.assign(c->ncells + 1, 0);
+  for (const auto& ch : c->chunks) c->cell_chunk_off[ch.cell + 1]++;
+  for (int k = 0; k < c->ncells; ++k) c->cell_chunk_off[k + 1] += c->cell_chunk_off[k];
+  c->cell_chunks.assign(c->chunks.size(), 0);
+  {
+    std::vector<int> fill(c->cell_chunk_off.begin(), c->cell_chunk_off.end() - 1);
+    for (size_t q = 0; q < c->chunks.size(); ++q) c->cell_chunks[fill[c->chunks[q].cell]++] = (int)q;
+  }
+  if (c->d_chunks.upload(c->chunks.data(), c->chunks.size()) ||
+      c->d_cell_chunk_off.upload(c->cell_chunk_off.data(), c->cell_chunk_off.size()) ||
+      c->d_cell_chunks.upload(c->cell_chunks.data(), c->cell_chunks.size()))
+    return fail(MZ_ERR_NOMEM, "chunk upload failed");
+  if (c->ncap > 0 && c->d_chunk_acc.alloc((size_t)c->ncap * c->chunks.size() * 8))
+    return fail(MZ_ERR_NOMEM, "chunk accumulator allocation failed");
   return MZ_OK;
 }
 
@@ -302,7 +315,8 @@ int ensure_eval_buffers(mz_ctx* c, int ncand) {
   const size_t nc = (size_t)ncand;
   c->resid_blocks = (c->natoms + c->ncells + 255) / 256;
   if (c->d_knots.alloc(nc * c->nknot) || c->d_tables.alloc(nc * c->table_bytes) ||
-      c->d_forces.alloc(nc * c->natoms * 3) || c->d_cell_acc.alloc(nc * c->ncells * 8) ||
+      c->d_forces.alloc(nc * c->natoms * 3) || c->d_tvec.alloc(nc * c->npairs * 3) ||
+      c->d_fown.alloc(nc * c->natoms * 3) || c->d_chunk_acc.alloc(nc * c->chunks.size() * 8) ||
       c->d_partial.alloc(nc * c->resid_blocks) || c->d_F.alloc(nc))
     return fail(MZ_ERR_NOMEM, "device allocation of evaluation buffers failed");
   c->ncap = ncand;
@@ -349,8 +363,10 @@ mz::AtomArgs atom_args(const mz_ctx* c) {
   a.natoms = c->natoms;
   a.ncells = c->ncells;
   a.npairs = c->npairs;
-  a.forces = c->d_forces.p;
-  a.cell_acc = c->d_cell_acc.p;
+  a.nchunks = (int)c->chunks.size();
+  a.tvec = c->d_tvec.p;
+  a.fown = c->d_fown.p;
+  a.chunk_acc = c->d_chunk_acc.p;
   return a;
 }
 
@@ -372,7 +388,10 @@ mz::ResidArgs resid_args(const mz_ctx* c) {
   r.energy_weight = c->energy_weight;
   r.stress_weight = c->stress_weight;
   r.forces = c->d_forces.p;
-  r.cell_acc = c->d_cell_acc.p;
+  r.chunk_acc = c->d_chunk_acc.p;
+  r.cell_chunk_off = c->d_cell_chunk_off.p;
+  r.cell_chunks = c->d_cell_chunks.p;
+  r.nchunks = (int)c->chunks.size();
   r.partial = c->d_partial.p;
   return r;
 }
@@ -411,8 +430,6 @@ int mz_run_forward(mz_ctx* c, int ncand, double* d_errvec, double* d_ce, double*
   if (c->timing) MZ_HIP(hipEventRecord(c->ev[0], st));
   mz::launch_refresh(st, c->d_knots.p, c->nknot, refresh_meta(c), c->d_tables.p, c->table_bytes, ncand);
   if (c->timing) MZ_HIP(hipEventRecord(c->ev[1], st));
-  mz::launch_zero(st, c->d_forces.p, (size_t)ncand * c->natoms * 3);
-  mz::launch_zero(st, c->d_cell_acc.p, (size_t)ncand * c->ncells * 8);
   if (c->timing) MZ_HIP(hipEventRecord(c->ev[2], st));
   mz::AtomArgs aa = atom_args(c);
   aa.rho_out = d_rho;
@@ -422,6 +439,15 @@ int mz_run_forward(mz_ctx* c, int ncand, double* d_errvec, double* d_ce, double*
   if (mz::launch_atoms(st, aa, (int)c->chunks.size(), ncand, c->tile.block, ppl))
     return fail(MZ_ERR_ARG, "unsupported tile (block, pairs per lane)");
   MZ_HIP(hipGetLastError());
+  mz::GatherArgs ga{};
+  ga.natoms = c->natoms;
+  ga.npairs = c->npairs;
+  ga.in_off = c->d_in_off.p;
+  ga.in_list = c->d_in_list.p;
+  ga.tvec = c->d_tvec.p;
+  ga.fown = c->d_fown.p;
+  ga.forces = c->d_forces.p;
+  mz::launch_gather(st, ga, ncand);
   if (c->timing) MZ_HIP(hipEventRecord(c->ev[3], st));
   mz::ResidArgs ra = resid_args(c);
   ra.errvec = d_errvec;
@@ -611,6 +637,16 @@ int mz_create(const mz_layout* layout, const mz_db* db, mz_ctx** out) {
   }
   c->ntriplets = 0;
   for (int a = 0; a < na; ++a) c->ntriplets += c->ntrip_atom[a];
+  // incoming pairs of every atom (counting sort of the pairs by neighbour):
+  // the force on atom j from pair (i -> j) is -T of that pair
+  c->in_off.assign(na + 1, 0);
+  for (long long q = 0; q < np; ++q) c->in_off[c->pairs[q].x + 1]++;
+  for (int a = 0; a < na; ++a) c->in_off[a + 1] += c->in_off[a];
+  c->in_list.assign(np, 0);
+  {
+    std::vector<int> fill(c->in_off.begin(), c->in_off.end() - 1);
+    for (long long q = 0; q < np; ++q) c->in_list[fill[c->pairs[q].x]++] = (int)q;
+  }
 
   // device upload
   if (hipGetDevice(&c->device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess) {
@@ -621,7 +657,8 @@ int mz_create(const mz_layout* layout, const mz_db* db, mz_ctx** out) {
   c->stress_weight = db->stress_weight;
   std::vector<double> s0(db->stress0, db->stress0 + 6 * (size_t)c->ncells);
   bool bad = c->d_atom_poff.upload(c->atom_poff.data(), c->atom_poff.size()) ||
-             c->d_pairs.upload(c->pairs.data(), c->pairs.size()) ||
+             c->d_pairs.upload(c->pairs.data(), c->pairs.size()) || c->d_in_off.upload(c->in_off.data(), c->in_off.size()) ||
+             c->d_in_list.upload(c->in_list.data(), c->in_list.size()) ||
              c->d_pos.upload(db->pos, 3 * (size_t)na) || c->d_box.upload(db->box, 9 * (size_t)c->ncells) ||
              c->d_types.upload(c->types.data(), na) || c->d_cell_of.upload(c->cell_of.data(), na) ||
              c->d_natoms_c.upload(c->natoms_c.data(), c->ncells) || c->d_abase.upload(c->abase.data(), c->ncells) ||
@@ -658,10 +695,10 @@ void mz_destroy(mz_ctx* c) {
   mz_adjoint_drop(c);
   DevBuf<double>* dbufs[] = {&c->d_pos, &c->d_box, &c->d_weight, &c->d_eweight, &c->d_sweight, &c->d_vol,
                              &c->d_energy0, &c->d_stress0, &c->d_force0, &c->d_x0, &c->d_xn, &c->d_knots,
-                             &c->d_forces, &c->d_cell_acc, &c->d_partial, &c->d_F, &c->d_errvec, &c->d_cell_e,
+                             &c->d_forces, &c->d_tvec, &c->d_fown, &c->d_chunk_acc, &c->d_partial, &c->d_F, &c->d_errvec, &c->d_cell_e,
                              &c->d_cell_s, &c->d_rho, &c->d_dF, &c->d_G};
   for (auto* b : dbufs) b->release();
-  DevBuf<int>* ibufs[] = {&c->d_atom_poff, &c->d_types, &c->d_cell_of, &c->d_natoms_c, &c->d_abase,
+  DevBuf<int>* ibufs[] = {&c->d_in_off, &c->d_in_list, &c->d_cell_chunk_off, &c->d_cell_chunks, &c->d_atom_poff, &c->d_types, &c->d_cell_of, &c->d_natoms_c, &c->d_abase,
                           &c->d_nknots, &c->d_basis, &c->d_koff, &c->d_bc, &c->d_coef_off};
   for (auto* b : ibufs) b->release();
   c->d_pairs.release();
@@ -786,9 +823,22 @@ int mz_ensure_buffers(mz_ctx* c, int ncand) { return ensure_eval_buffers(c, ncan
 int mz_batch_impl(mz_ctx* c, int ncand, const double* knots, double* F, double* G) {
   if (!c || !knots || !F || ncand < 1) return fail(MZ_ERR_ARG, "null argument or ncand < 1");
   if (G) return mz_grad_impl(c, ncand, knots, F, G);
-  int rc = ensure_eval_buffers(c, ncand);
+  // candidates go through in sub-batches that keep the per-candidate pair
+  // buffers within a fixed share of HBM
+  const size_t per = (size_t)c->npairs * 24 + (size_t)c->natoms * 72 + c->chunks.size() * 64 +
+                     (size_t)c->nknot * 8 + (size_t)c->table_bytes + 64;
+  const int sub = (int)std::max<size_t>(1, std::min<size_t>((size_t)ncand, (48ull << 30) / per));
+  int rc = ensure_eval_buffers(c, sub);
   if (rc) return rc;
-  if (!c->timing) return mz_forward_graph(c, ncand, knots, F);
+  if (!c->timing) {
+    for (int c0 = 0; c0 < ncand; c0 += sub) {
+      const int nb = std::min(sub, ncand - c0);
+      rc = mz_forward_graph(c, nb, knots + (size_t)c0 * c->nknot, F + c0);
+      if (rc) return rc;
+    }
+    return MZ_OK;
+  }
+  if (sub < ncand) return fail(MZ_ERR_CAPACITY, "timed batch larger than one sub-batch");
   rc = mz_stage_knots(c, ncand, knots);
   if (rc) return rc;
   rc = mz_run_forward(c, ncand, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);

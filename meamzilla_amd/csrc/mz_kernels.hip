@@ -220,7 +220,7 @@ struct AtomSmem {
   double dF[MAXA];
   double fi[MAXA][3];
   int poff[MAXA + 1];
-  double red[BLOCK / 64];
+  double red8[8][BLOCK / 64];
 };
 
 template <int BLOCK, int PPL, int MAXA, bool MEAM, bool POINT, bool Q>
@@ -248,7 +248,6 @@ __global__ void __launch_bounds__(BLOCK) k_atoms(const AtomArgs args) {
 
   const int nt = args.pot.nt;
   const double* box = args.box + 9 * ch.cell;
-  double* forces = args.forces + (size_t)cand * args.natoms * 3;
 
   // per-lane pair state
   int ia[PPL], jg[PPL];
@@ -429,10 +428,10 @@ __global__ void __launch_bounds__(BLOCK) k_atoms(const AtomArgs args) {
       g[1] = Gy[s];
       g[2] = Gz[s];
     }
-    double* fj = forces + 3 * (size_t)jg[s];
-    unsafeAtomicAdd(fj + 0, -tx);
-    unsafeAtomicAdd(fj + 1, -ty);
-    unsafeAtomicAdd(fj + 2, -tz);
+    double* tv = args.tvec + ((size_t)cand * args.npairs + ch.p0 + tid + s * BLOCK) * 3;
+    tv[0] = tx;
+    tv[1] = ty;
+    tv[2] = tz;
     atomicAdd(&sm.fi[ia[s]][0], tx);
     atomicAdd(&sm.fi[ia[s]][1], ty);
     atomicAdd(&sm.fi[ia[s]][2], tz);
@@ -446,18 +445,51 @@ __global__ void __launch_bounds__(BLOCK) k_atoms(const AtomArgs args) {
   }
   __syncthreads();
   for (int a = tid; a < ch.na; a += BLOCK) {
-    double* fa = forces + 3 * (size_t)(ch.a0 + a);
-    unsafeAtomicAdd(fa + 0, sm.fi[a][0]);
-    unsafeAtomicAdd(fa + 1, sm.fi[a][1]);
-    unsafeAtomicAdd(fa + 2, sm.fi[a][2]);
+    double* fa = args.fown + ((size_t)cand * args.natoms + ch.a0 + a) * 3;
+    fa[0] = sm.fi[a][0];
+    fa[1] = sm.fi[a][1];
+    fa[2] = sm.fi[a][2];
   }
-  double* cacc = args.cell_acc + ((size_t)cand * args.ncells + ch.cell) * 8;
+  // chunk partials of energy, virial and penalty: one wave-level reduction
+  // per value, then the per-wave sums in fixed order
   const double vals[8] = {e_acc, sxx, syy, szz, sxy, syz, szx, c_acc};
+  const int lane = tid & 63, wid = tid >> 6;
 #pragma unroll
   for (int q = 0; q < 8; ++q) {
-    const double t = block_sum<BLOCK>(vals[q], sm.red);
-    if (tid == 0) unsafeAtomicAdd(cacc + q, t);
+    const double v = wave_sum(vals[q]);
+    if (lane == 0) sm.red8[q][wid] = v;
   }
+  __syncthreads();
+  if (tid < 8) {
+    double t = 0.0;
+#pragma unroll
+    for (int w = 0; w < BLOCK / 64; ++w) t += sm.red8[tid][w];
+    args.chunk_acc[((size_t)cand * args.nchunks + blockIdx.x) * 8 + tid] = t;
+  }
+}
+
+// ------------------------------------------------------------------ k_gather
+
+// One thread per atom; every pair pointing at the atom carries (minus) the
+// force its owner exerted on it.  Fixed summation order, no atomics.
+__global__ void __launch_bounds__(256) k_gather(const GatherArgs a) {
+  const int cand = blockIdx.y;
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= a.natoms) return;
+  const double* tv = a.tvec + (size_t)cand * a.npairs * 3;
+  const double* fo = a.fown + ((size_t)cand * a.natoms + i) * 3;
+  double fx = fo[0], fy = fo[1], fz = fo[2];
+  const int b = a.in_off[i], e = a.in_off[i + 1];
+  for (int q = b; q < e; ++q) {
+    const double* t = tv + 3 * (size_t)a.in_list[q];
+    fx -= t[0];
+    fy -= t[1];
+    fz -= t[2];
+  }
+  double* f = a.forces + ((size_t)cand * a.natoms + i) * 3;
+  f[0] = fx;
+  f[1] = fy;
+  f[2] = fz;
 }
 
 // ------------------------------------------------------------------ k_resid
@@ -469,7 +501,7 @@ __global__ void __launch_bounds__(BLOCK) k_resid(const ResidArgs a) {
   const int idx = blockIdx.x * BLOCK + threadIdx.x;
   double sq = 0.0;
   double* ev = a.errvec ? a.errvec + (size_t)cand * a.nerr : nullptr;
-  const double* cacc = a.cell_acc + (size_t)cand * a.ncells * 8;
+  const double* cacc_all = a.chunk_acc + (size_t)cand * a.nchunks * 8;
   if (idx < a.natoms) {
     const int c = a.cell_of[idx];
     const double w = a.weight[c];
@@ -485,7 +517,13 @@ __global__ void __launch_bounds__(BLOCK) k_resid(const ResidArgs a) {
   } else if (idx < a.natoms + a.ncells) {
     const int c = idx - a.natoms;
     const int n = a.natoms_c[c];
-    const double E = cacc[8 * c] / n;
+    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int q = a.cell_chunk_off[c]; q < a.cell_chunk_off[c + 1]; ++q) {
+      const double* cp = cacc_all + 8 * (size_t)a.cell_chunks[q];
+#pragma unroll
+      for (int m = 0; m < 8; ++m) acc[m] += cp[m];
+    }
+    const double E = acc[0] / n;
     const size_t e0 = 7 * (size_t)c + 3 * (size_t)a.abase[c];
     const double WE = a.weight[c] * a.eweight[c] * a.energy_weight;
     const double eE = WE * n * (E - a.energy0[c]);
@@ -496,7 +534,7 @@ __global__ void __launch_bounds__(BLOCK) k_resid(const ResidArgs a) {
     if (a.cell_energy) a.cell_energy[(size_t)cand * a.ncells + c] = E;
 #pragma unroll
     for (int m = 0; m < 6; ++m) {
-      const double S = cacc[8 * c + 1 + m] / a.vol[c];
+      const double S = acc[1 + m] / a.vol[c];
       const double WS = a.weight[c] * a.sweight[c] * a.stress_weight;
       const double es = WS * (S - a.stress0[6 * c + m]);
       sq += es * es;
@@ -504,7 +542,7 @@ __global__ void __launch_bounds__(BLOCK) k_resid(const ResidArgs a) {
       if (ev) ev[e0 + 1 + m] = es;
       if (a.cell_stress) a.cell_stress[((size_t)cand * a.ncells + c) * 6 + m] = S;
     }
-    const double con = cacc[8 * c + 7];
+    const double con = acc[7];
     sq += con * con;
     if (adj) adj[7] = 2.0 * con;
     if (ev) ev[7 * (size_t)a.ncells + 3 * (size_t)a.natoms + c] = con;
@@ -601,6 +639,10 @@ int launch_atoms(hipStream_t st, const AtomArgs& a, int nchunks, int ncand, int 
   else if (block == 64 && ppl == 4) launch_atoms_t<64, 4, 64>(st, a, nchunks, ncand);
   else return -1;
   return 0;
+}
+
+void launch_gather(hipStream_t st, const GatherArgs& a, int ncand) {
+  hipLaunchKernelGGL(k_gather, dim3((a.natoms + 255) / 256, ncand), dim3(256), 0, st, a);
 }
 
 void launch_resid(hipStream_t st, const ResidArgs& a, int nblocks, int ncand) {
