@@ -143,7 +143,7 @@ int mz_autotune(mz_ctx* ctx, int ncand, int reps, mz_tile* best);
  * timing enabled (HIP events on the context stream). */
 typedef struct {
   double atom_ms;       /* atom (pair + triplet + embedding + force) kernel */
-  double resid_ms;      /* residual + reduction kernels */
+  double resid_ms;      /* force gather, residual and reduction kernels */
   double refresh_ms;    /* spline refresh kernel */
   double adjoint_ms;    /* reverse sweep (gradient) kernel, if run */
   double total_ms;      /* whole evaluation on the stream */

@@ -59,6 +59,8 @@ struct PotDesc {
   int32_t table_bytes;  // heads + intervals of one table
   int32_t nknot;        // knot-vector length
   int32_t quintic;      // any spline5eq present
+  int32_t gco0;         // first interval record of the g splines (MEAM)
+  int32_t ngco;         // interval records of the g splines
 };
 
 // Packs an image triple into one 32-bit word (8 bits each, signed).

@@ -99,6 +99,8 @@ int build_layout(mz_ctx* c, const mz_layout* L) {
   c->pot.table_bytes = c->table_bytes;
   c->pot.nknot = koff;
   c->pot.quintic = quintic;
+  c->pot.gco0 = c->meam ? c->coef_off[c->pot.fn_off[4]] : coff;
+  c->pot.ngco = c->meam ? coff - c->pot.gco0 : 0;
   return MZ_OK;
 }
 
@@ -439,6 +441,7 @@ int mz_run_forward(mz_ctx* c, int ncand, double* d_errvec, double* d_ce, double*
   if (mz::launch_atoms(st, aa, (int)c->chunks.size(), ncand, c->tile.block, ppl))
     return fail(MZ_ERR_ARG, "unsupported tile (block, pairs per lane)");
   MZ_HIP(hipGetLastError());
+  if (c->timing) MZ_HIP(hipEventRecord(c->ev[7], st));
   mz::GatherArgs ga{};
   ga.natoms = c->natoms;
   ga.npairs = c->npairs;
@@ -477,7 +480,7 @@ void mz_record_timing(mz_ctx* c) {
   if (!c->timing) return;
   float t01 = 0, t23 = 0, t34 = 0, t04 = 0;
   (void)hipEventElapsedTime(&t01, c->ev[0], c->ev[1]);
-  (void)hipEventElapsedTime(&t23, c->ev[2], c->ev[3]);
+  (void)hipEventElapsedTime(&t23, c->ev[2], c->ev[7]);
   (void)hipEventElapsedTime(&t34, c->ev[3], c->ev[4]);
   (void)hipEventElapsedTime(&t04, c->ev[0], c->ev[4]);
   c->last.refresh_ms = t01;
@@ -783,6 +786,10 @@ int mz_set_tile(mz_ctx* c, const mz_tile* t) {
   const bool ok = (t->block == 256 && (ppl == 1 || ppl == 2)) || (t->block == 128 && (ppl == 2 || ppl == 4)) ||
                   (t->block == 512 && ppl == 1) || (t->block == 64 && ppl == 4);
   if (!ok || ppl * t->block != t->max_pairs) return fail(MZ_ERR_ARG, "unsupported tile (block, max_pairs)");
+  // static LDS of the atom kernel: 72 bytes per pair slot plus ~3.5 KB, and
+  // the staged spline tables
+  const size_t lds = (size_t)t->max_pairs * 72 + 3584 + c->table_bytes + (size_t)c->pot.ngco * 48;
+  if (lds > 64 * 1024) return fail(MZ_ERR_CAPACITY, "tile and spline tables exceed 64 KB of LDS");
   mz_tile old = c->tile;
   drop_graphs(c);
   c->tile = *t;

@@ -215,7 +215,8 @@ __global__ void k_zero(double* __restrict__ p, size_t n) {
 // LDS layout of one workgroup (static part).
 template <int BLOCK, int PPL, int MAXA>
 struct AtomSmem {
-  double4 kd[BLOCK * PPL];   // per pair: unit vector and f
+  double kd[BLOCK * PPL][6];   // per pair: unit vector, f, f', 1/r (48-byte records)
+  double gl[3][BLOCK * PPL];   // three-body leg sums pushed by the partner lanes
   double rho[MAXA];
   double dF[MAXA];
   double fi[MAXA][3];
@@ -226,12 +227,14 @@ struct AtomSmem {
 template <int BLOCK, int PPL, int MAXA, bool MEAM, bool POINT, bool Q>
 __global__ void __launch_bounds__(BLOCK) k_atoms(const AtomArgs args) {
   extern __shared__ __attribute__((aligned(64))) char dyn[];
-  __shared__ AtomSmem<BLOCK, PPL, MAXA> sm;
+  __shared__ __attribute__((aligned(16))) AtomSmem<BLOCK, PPL, MAXA> sm;
   const int cand = blockIdx.y;
   const Chunk ch = args.chunks[blockIdx.x];
   const int tid = threadIdx.x;
 
-  // stage this candidate's spline table in LDS
+  // stage this candidate's spline table in LDS, plus the g intervals again as
+  // 48-byte records (6 coefficients) so that lanes reading different
+  // intervals hit different banks
   {
     const int4* src = reinterpret_cast<const int4*>(args.tables + (size_t)cand * args.table_bytes);
     int4* dst = reinterpret_cast<int4*>(dyn);
@@ -239,6 +242,12 @@ __global__ void __launch_bounds__(BLOCK) k_atoms(const AtomArgs args) {
   }
   const SplHead* HD = reinterpret_cast<const SplHead*>(dyn);
   const SplInterval* IV = reinterpret_cast<const SplInterval*>(dyn + sizeof(SplHead) * args.pot.nspl);
+  double* gt = reinterpret_cast<double*>(dyn + args.table_bytes);
+  if (MEAM) {
+    const SplInterval* src = reinterpret_cast<const SplInterval*>(args.tables + (size_t)cand * args.table_bytes +
+                                                                  sizeof(SplHead) * args.pot.nspl);
+    for (int q = tid; q < args.pot.ngco * 6; q += BLOCK) gt[q] = src[args.pot.gco0 + q / 6].c[q % 6];
+  }
   for (int a = tid; a <= ch.na; a += BLOCK) sm.poff[a] = args.atom_poff[ch.a0 + a] - ch.p0;
   for (int a = tid; a < ch.na; a += BLOCK) {
     sm.rho[a] = 0.0;
@@ -315,44 +324,50 @@ __global__ void __launch_bounds__(BLOCK) k_atoms(const AtomArgs args) {
           fv[s] = v;
           dfv[s] = dv;
         }
-        sm.kd[p] = make_double4(ux[s], uy[s], uz[s], fv[s]);
+        double2* K = reinterpret_cast<double2*>(sm.kd[p]);
+        K[0] = make_double2(ux[s], uy[s]);
+        K[1] = make_double2(uz[s], fv[s]);
+        K[2] = make_double2(dfv[s], invr[s]);
+        sm.gl[0][p] = sm.gl[1][p] = sm.gl[2][p] = 0.0;
       }
     }
   }
   __syncthreads();
 
-  // ---- phase 2: triplets (pull form, dF factored out)
+  // ---- phase 2: triplets, each unordered (j, k) once (dF factored out)
+  // Lane j of an atom with n pairs visits partners k = j+1 .. j+(n-1)/2
+  // (mod n), plus k = j+n/2 for the first n/2 lanes when n is even: every
+  // lane gets the same number of partners.  The own leg accumulates in
+  // registers, the partner leg is pushed into LDS (distinct k per lane and
+  // step, so the atomics never collide).
   if (MEAM) {
 #pragma unroll
     for (int s = 0; s < PPL; ++s) {
       const int p = tid + s * BLOCK;
       if (ia[s] < 0) continue;
       const int kb = sm.poff[ia[s]], ke = sm.poff[ia[s] + 1];
+      const int n = ke - kb, l = p - kb;
+      const int trips = ((n - 1) >> 1) + (((n & 1) == 0 && l < (n >> 1)) ? 1 : 0);
       const int ti = args.types[ch.a0 + ia[s]];
       const SplHead& Hg = HD[args.pot.fn_off[4] + ti];
-      // g header in registers for the whole sweep
       const double gx0 = Hg.x0, ginv = Hg.invstep, gxcut = Hg.xcut, gxn = Hg.xn;
       const double gkmax = (double)(Hg.n - 2);
       const double gy0 = Hg.y0, gyn = Hg.yn, ggL = Hg.gL, ggR = Hg.gR, gg2L = Hg.g2L, gg2R = Hg.g2R;
-      const SplInterval* IVg = IV + Hg.coef_off;
-      const double fj = fv[s], dfj = dfv[s];
-      const double ivg = invr[s] * ginv;  // invr_j folded with the t -> cos scale
-      const double ivj = invr[s];
+      const double* gtg = gt + (size_t)(Hg.coef_off - args.pot.gco0) * 6;
+      const double fj = fv[s], dfj = dfv[s], ivj = invr[s];
       const double xj = ux[s], yj = uy[s], zj = uz[s];
       double rho_t = 0.0, gx = 0.0, gy = 0.0, gz = 0.0;
-      // Every k of atom i is visited; the own pair and (knot style) partners
-      // with cos beyond x_n (1 + 1e-8) get f_k = 0, which turns their terms
-      // into exact zeros.  Legs beyond their f cutoff already carry f = f' = 0,
-      // so the triplets the point style drops (src/pots/pot_meam/triplet_meam.cpp:64-65)
-      // add zeros as well.
-#pragma unroll 2
-      for (int k = kb; k < ke; ++k) {
-        const double4 dk = sm.kd[k];
-        const double cs = xj * dk.x + yj * dk.y + zj * dk.z;
+      int kl = l;
+      for (int m = 0; m < trips; ++m) {
+        kl = (kl + 1 == n) ? 0 : kl + 1;
+        const int k = kb + kl;
+        const double2* K = reinterpret_cast<const double2*>(sm.kd[k]);
+        const double2 ka = K[0], kbv = K[1], kc = K[2];
+        const double xk = ka.x, yk = ka.y, zk = kbv.x;
+        const double cs = xj * xk + yj * yk + zj * zk;
         double gv, dgt;  // g and dg/dt (t = shift within the interval)
-        bool keep = k != p;
+        bool keep = true;
         if (POINT && (cs < gx0 || cs > gxn)) {
-          // linear (cubic) or quadratic (quintic) extrapolation of g
           const bool left = cs < gx0;
           const double e = cs - (left ? gx0 : gxn);
           const double g1 = left ? ggL : ggR, g2 = left ? gg2L : gg2R;
@@ -362,32 +377,48 @@ __global__ void __launch_bounds__(BLOCK) k_atoms(const AtomArgs args) {
           const double u = (cs - gx0) * ginv;
           const double fl = fmin(fmax(floor(u), 0.0), gkmax);
           const double t = u - fl;
-          const SplInterval& I = IVg[(int)fl];
+          const double2* C = reinterpret_cast<const double2*>(gtg + 6 * (int)fl);
+          const double2 c01 = C[0], c23 = C[1];
           if (Q) {
-            gv = I.c[0] + t * (I.c[1] + t * (I.c[2] + t * (I.c[3] + t * (I.c[4] + t * I.c[5]))));
-            dgt = I.c[1] + t * (2.0 * I.c[2] + t * (3.0 * I.c[3] + t * (4.0 * I.c[4] + t * 5.0 * I.c[5])));
+            const double2 c45 = C[2];
+            gv = c01.x + t * (c01.y + t * (c23.x + t * (c23.y + t * (c45.x + t * c45.y))));
+            dgt = c01.y + t * (2.0 * c23.x + t * (3.0 * c23.y + t * (4.0 * c45.x + t * 5.0 * c45.y)));
           } else {
-            gv = I.c[0] + t * (I.c[1] + t * (I.c[2] + t * I.c[3]));
-            dgt = I.c[1] + t * (2.0 * I.c[2] + t * 3.0 * I.c[3]);
+            gv = c01.x + t * (c01.y + t * (c23.x + t * c23.y));
+            dgt = c01.y + t * (2.0 * c23.x + t * 3.0 * c23.y);
           }
-          if (!POINT) keep = keep && (cs < gxcut);
+          if (!POINT) keep = cs < gxcut;
         }
-        const double fk = keep ? dk.w : 0.0;
+        const double fk = keep ? kbv.y : 0.0;
+        const double dfk = keep ? kc.x : 0.0;
+        const double ivk = kc.y;
         const double ff = fj * fk;
         rho_t += ff * gv;
-        const double vl = dgt * ff * ivg;            // V3 / dF * invr_j
-        const double vv = gv * dfj * fk - vl * cs;   // (dV3j - vlj cos) / dF
-        gx += xj * vv + dk.x * vl;
-        gy += yj * vv + dk.y * vl;
-        gz += zj * vv + dk.z * vl;
+        const double V3 = dgt * ginv * ff;          // V3 / dF
+        const double vlj = V3 * ivj, vlk = V3 * ivk;
+        const double vvj = gv * dfj * fk - vlj * cs;  // (dV3j - vlj cos) / dF
+        const double vvk = gv * fj * dfk - vlk * cs;  // (dV3k - vlk cos) / dF
+        gx += xj * vvj + xk * vlj;
+        gy += yj * vvj + yk * vlj;
+        gz += zj * vvj + zk * vlj;
+        atomicAdd(&sm.gl[0][k], xk * vvk + xj * vlk);
+        atomicAdd(&sm.gl[1][k], yk * vvk + yj * vlk);
+        atomicAdd(&sm.gl[2][k], zk * vvk + zj * vlk);
       }
-      (void)ivj;
       Gx[s] = gx;
       Gy[s] = gy;
       Gz[s] = gz;
-      atomicAdd(&sm.rho[ia[s]], 0.5 * rho_t);
+      atomicAdd(&sm.rho[ia[s]], rho_t);
     }
     __syncthreads();
+#pragma unroll
+    for (int s = 0; s < PPL; ++s) {
+      const int p = tid + s * BLOCK;
+      if (ia[s] < 0) continue;
+      Gx[s] += sm.gl[0][p];
+      Gy[s] += sm.gl[1][p];
+      Gz[s] += sm.gl[2][p];
+    }
   }
 
   // ---- phase 3: embedding
@@ -616,7 +647,7 @@ static void launch_atoms_t(hipStream_t st, const AtomArgs& a, int nchunks, int n
   const bool meam = style == kMeamSpline || style == kMeam;
   const bool point = style == kEam || style == kMeam;
   const bool q = a.pot.quintic != 0;
-  const size_t dyn = (size_t)a.table_bytes;
+  const size_t dyn = (size_t)a.table_bytes + (size_t)a.pot.ngco * 48;
   dim3 grid(nchunks, ncand), block(BLOCK);
 #define MZ_LAUNCH(M, P, Q_)                                                                          \
   hipLaunchKernelGGL((k_atoms<BLOCK, PPL, MAXA, M, P, Q_>), grid, block, dyn, st, a)
