@@ -33,7 +33,6 @@ namespace mz {
 
 constexpr int kAdjBlock = 256;   // threads (= pairs) per adjoint workgroup
 constexpr int kAdjAtoms = 64;    // atoms per adjoint chunk
-constexpr int kGReps = 4;        // LDS replicas of the g-interval adjoints
 constexpr int kSlots = 64;       // global partial tables per candidate
 
 struct AdjArgs {
@@ -56,6 +55,7 @@ struct AdjArgs {
   double* slots;           // [cand][kSlots][ntab]
   int ntab;                // ncoef*6 + nspl*6
   int gco0, ngco;          // first g interval record, number of g interval records
+  int grep_log2;           // log2 of the lane replicas of the g-coefficient adjoints
 };
 
 struct ChainArgs {
@@ -78,8 +78,8 @@ struct ChainArgs {
 
 struct Partner {
   double4 a;  // unit vector, f
-  double4 b;  // a f', invr f w (3)
-  double2 c;  // invr f a, unused
+  double4 b;  // a f', w / r (3)
+  double2 c;  // a / r, unused
 };
 
 // Interval-coefficient adjoint: weight al on the value, bt on d(value)/dt.
@@ -157,12 +157,15 @@ template <bool MEAM, bool POINT, bool Q>
 __global__ void __launch_bounds__(kAdjBlock) k_adjoint(const AdjArgs A) {
   extern __shared__ __attribute__((aligned(64))) char dyn[];
   __shared__ Partner pd[kAdjBlock];
+  __shared__ double push[2][kAdjBlock];
   __shared__ double S[kAdjAtoms], psi[kAdjAtoms], Up[kAdjAtoms];
   __shared__ int poff[kAdjAtoms + 1];
+  constexpr int NC = Q ? 6 : 4;  // coefficients per g interval that carry adjoints
   const int cand = blockIdx.y;
   const int tid = threadIdx.x;
   const Chunk ch = A.chunks[blockIdx.x];
   const int ncoef = A.pot.ncoef;
+  const int R = 1 << A.grep_log2;
   {
     const int4* src = reinterpret_cast<const int4*>(A.tables + (size_t)cand * A.table_bytes);
     int4* dst = reinterpret_cast<int4*>(dyn);
@@ -170,10 +173,16 @@ __global__ void __launch_bounds__(kAdjBlock) k_adjoint(const AdjArgs A) {
   }
   const SplHead* HD = reinterpret_cast<const SplHead*>(dyn);
   const SplInterval* IV = reinterpret_cast<const SplInterval*>(dyn + sizeof(SplHead) * A.pot.nspl);
-  double* atab = reinterpret_cast<double*>(dyn + A.table_bytes);
-  double* grep = atab + A.ntab;
-  const int ngrep = MEAM ? kGReps * A.ngco * 6 : 0;
-  for (int q = tid; q < A.ntab + ngrep; q += kAdjBlock) atab[q] = 0.0;
+  double* gt = reinterpret_cast<double*>(dyn + A.table_bytes);   // g coefficients, 48-byte records
+  double* atab = gt + (MEAM ? A.ngco * 6 : 0);                    // coefficient + head adjoints
+  double* gacc = atab + A.ntab;                                   // [g record][coef][replica]
+  if (MEAM) {
+    const SplInterval* src = reinterpret_cast<const SplInterval*>(A.tables + (size_t)cand * A.table_bytes +
+                                                                  sizeof(SplHead) * A.pot.nspl);
+    for (int q = tid; q < A.ngco * 6; q += kAdjBlock) gt[q] = src[A.gco0 + q / 6].c[q % 6];
+  }
+  const int nacc = A.ntab + (MEAM ? A.ngco * NC * R : 0);
+  for (int q = tid; q < nacc; q += kAdjBlock) atab[q] = 0.0;
   for (int a = tid; a <= ch.na; a += kAdjBlock) poff[a] = A.atom_poff[ch.a0 + a] - ch.p0;
   for (int a = tid; a < ch.na; a += kAdjBlock) S[a] = 0.0;
   __syncthreads();
@@ -239,9 +248,10 @@ __global__ void __launch_bounds__(kAdjBlock) k_adjoint(const AdjArgs A) {
       sv += wx * G[0] + wy * G[1] + wz * G[2];
       Partner P;
       P.a = make_double4(ux, uy, uz, fj);
-      P.b = make_double4(aj * dfj, invr * fj * wx, invr * fj * wy, invr * fj * wz);
-      P.c = make_double2(invr * fj * aj, 0.0);
+      P.b = make_double4(aj * dfj, invr * wx, invr * wy, invr * wz);
+      P.c = make_double2(invr * aj, 0.0);
       pd[p] = P;
+      push[0][p] = push[1][p] = 0.0;
     }
     atomicAdd(&S[ia], sv);
   }
@@ -267,26 +277,43 @@ __global__ void __launch_bounds__(kAdjBlock) k_adjoint(const AdjArgs A) {
   }
   __syncthreads();
 
-  // ---- phase C: ordered triplets (j = own pair, k = partner)
+  // ---- phase C: unordered triplets {j, k} of each atom.
+  // With B_jk = (w_j . d_k - a_j cos) / r_j and X = psi g + U' g' (B_jk + B_kj),
+  // one visit adds f_k X + U' g A_k to the f-value adjoint of pair j and
+  // f_j X + U' g A_j to that of pair k (A = a f'), g f_k and g f_j to their
+  // f'-adjoint sums, and the g-coefficient adjoint of weights
+  // psi f_j f_k + U' (A_j f_k + A_k f_j) (value) and U' f_j f_k (B_jk + B_kj)
+  // (slope).  The lane of pair j visits partners j+1 .. j+(n-1)/2 (mod n),
+  // plus j+n/2 for the first n/2 lanes when n is even (the forward pass's
+  // round-robin); the partner sums go through LDS atomics on distinct k.
   double Phi = 0.0, Phip = 0.0;
   if (MEAM && has) {
     const double ps = psi[ia], up = Up[ia];
     const int kb = poff[ia], ke = poff[ia + 1];
+    const int n = ke - kb, l = p - kb;
+    const int trips = ((n - 1) >> 1) + (((n & 1) == 0 && l < (n >> 1)) ? 1 : 0);
     const int sg = A.pot.fn_off[4] + ti;
     const SplHead& Hg = HD[sg];
     const double gx0 = Hg.x0, ginv = Hg.invstep, gxcut = Hg.xcut, gxn = Hg.xn;
     const double gkmax = (double)(Hg.n - 2);
     const double gy0 = Hg.y0, gyn = Hg.yn, ggL = Hg.gL, ggR = Hg.gR, gg2L = Hg.g2L, gg2R = Hg.g2R;
     const bool gq = Hg.basis == 5;
-    const SplInterval* IVg = IV + Hg.coef_off;
-    double* gtab = grep + (tid & (kGReps - 1)) * A.ngco * 6;
-    const int gbase = Hg.coef_off - A.gco0;
+    const int grec = Hg.coef_off - A.gco0;
+    const double* gtg = gt + (size_t)grec * 6;
+    double* gacl = gacc + (size_t)grec * NC * R + (tid & (R - 1));
     double* ghead = atab + ncoef * 6 + sg * 6;
-    for (int k = kb; k < ke; ++k) {
-      const Partner P = pd[k];
-      const double cs = ux * P.a.x + uy * P.a.y + uz * P.a.z;
-      double gv, dgt, t = 0.0, fl = 0.0;
-      bool extrap = false, keep = k != p;
+    const double Aj = aj * dfj, Wjx = invr * wx, Wjy = invr * wy, Wjz = invr * wz, Cj = invr * aj;
+    double Rj = 0.0, Qj = 0.0;
+    int kl = l;
+    for (int m = 0; m < trips; ++m) {
+      kl = (kl + 1 == n) ? 0 : kl + 1;
+      const int k = kb + kl;
+      const double4 Pa = pd[k].a, Pb = pd[k].b;
+      const double Ck = pd[k].c.x;
+      const double cs = ux * Pa.x + uy * Pa.y + uz * Pa.z;
+      double gv, dgt, t;
+      int fl;
+      bool extrap = false, keep = true;
       if (POINT && (cs < gx0 || cs > gxn)) {
         extrap = true;
         const bool left = cs < gx0;
@@ -295,42 +322,68 @@ __global__ void __launch_bounds__(kAdjBlock) k_adjoint(const AdjArgs A) {
         gv = (left ? gy0 : gyn) + g1 * e + 0.5 * g2 * e * e;
         dgt = (g1 + g2 * e) / ginv;
         t = e;
-        fl = left ? 0.0 : 1.0;
+        fl = left ? 0 : 1;
       } else {
         const double u = (cs - gx0) * ginv;
-        fl = fmin(fmax(floor(u), 0.0), gkmax);
-        t = u - fl;
-        const SplInterval& I = IVg[(int)fl];
+        const double flo = fmin(fmax(floor(u), 0.0), gkmax);
+        t = u - flo;
+        fl = (int)flo;
+        const double2* C = reinterpret_cast<const double2*>(gtg + 6 * fl);
+        const double2 c01 = C[0], c23 = C[1];
         if (Q) {
-          gv = I.c[0] + t * (I.c[1] + t * (I.c[2] + t * (I.c[3] + t * (I.c[4] + t * I.c[5]))));
-          dgt = I.c[1] + t * (2.0 * I.c[2] + t * (3.0 * I.c[3] + t * (4.0 * I.c[4] + t * 5.0 * I.c[5])));
+          const double2 c45 = C[2];
+          gv = c01.x + t * (c01.y + t * (c23.x + t * (c23.y + t * (c45.x + t * c45.y))));
+          dgt = c01.y + t * (2.0 * c23.x + t * (3.0 * c23.y + t * (4.0 * c45.x + t * 5.0 * c45.y)));
         } else {
-          gv = I.c[0] + t * (I.c[1] + t * (I.c[2] + t * I.c[3]));
-          dgt = I.c[1] + t * (2.0 * I.c[2] + t * 3.0 * I.c[3]);
+          gv = c01.x + t * (c01.y + t * (c23.x + t * c23.y));
+          dgt = c01.y + t * (2.0 * c23.x + t * 3.0 * c23.y);
         }
-        if (!POINT) keep = keep && (cs < gxcut);
+        if (!POINT) keep = cs < gxcut;
       }
+      if (!keep) {
+        gv = 0.0;
+        dgt = 0.0;
+      }
+      const double fk = keep ? Pa.w : 0.0;
+      const double Ak = keep ? Pb.x : 0.0;
       const double gp = dgt * ginv;
-      const double fk = keep ? P.a.w : 0.0;
-      const double Ak = keep ? P.b.x : 0.0;
-      const double Wkx = keep ? P.b.y : 0.0, Wky = keep ? P.b.z : 0.0, Wkz = keep ? P.b.w : 0.0;
-      const double Ck = keep ? P.c.x : 0.0;
-      const double B = invr * (wx * P.a.x + wy * P.a.y + wz * P.a.z - aj * cs);
+      const double Bs = (Wjx * Pa.x + Wjy * Pa.y + Wjz * Pa.z - Cj * cs) + (Pb.y * ux + Pb.z * uy + Pb.w * uz - Ck * cs);
+      const double X = ps * gv + up * gp * Bs;
       const double ff = fj * fk;
-      const double al = 0.5 * ps * ff + up * aj * dfj * fk;
-      const double be = up * B * ff;
-      Phi += ps * fk * gv + up * (B * gp * fk + gv * Ak + gp * (Wkx * ux + Wky * uy + Wkz * uz - Ck * cs));
-      Phip += up * aj * gv * fk;
+      Rj += fk * X + up * gv * Ak;
+      Qj += gv * fk;
+      atomicAdd(&push[0][k], fj * X + up * gv * Aj);
+      atomicAdd(&push[1][k], gv * fj);
+      const double al = ps * ff + up * (Aj * fk + Ak * fj);
+      const double be = up * ff * Bs;
       if (POINT && extrap) {
-        const bool left = fl == 0.0;
+        const bool left = fl == 0;
         const double e = t;
         atomicAdd(ghead + (left ? 0 : 1), al);
         atomicAdd(ghead + (left ? 2 : 3), al * e + be);
         if (gq) atomicAdd(ghead + (left ? 4 : 5), 0.5 * al * e * e + be * e);
       } else {
-        acc_interval<Q>(gtab, gbase + (int)fl, t, al, be * ginv);
+        const double bt = be * ginv;
+        const double t2 = t * t, t3 = t2 * t;
+        double* cp = gacl + (size_t)fl * NC * R;
+        atomicAdd(cp, al);
+        atomicAdd(cp + R, al * t + bt);
+        atomicAdd(cp + 2 * R, al * t2 + 2.0 * bt * t);
+        atomicAdd(cp + 3 * R, al * t3 + 3.0 * bt * t2);
+        if (Q) {
+          const double t4 = t3 * t;
+          atomicAdd(cp + 4 * R, al * t4 + 4.0 * bt * t3);
+          atomicAdd(cp + 5 * R, al * t4 * t + 5.0 * bt * t4);
+        }
       }
     }
+    Phi = Rj;
+    Phip = Qj;
+  }
+  if (MEAM) __syncthreads();
+  if (MEAM && has) {
+    Phi += push[0][p];
+    Phip = Up[ia] * aj * (Phip + push[1][p]);
   }
 
   // ---- phase D: pair functions
@@ -346,11 +399,10 @@ __global__ void __launch_bounds__(kAdjBlock) k_adjoint(const AdjArgs A) {
   }
   __syncthreads();
   if (MEAM) {
-    for (int q = tid; q < A.ngco * 6; q += kAdjBlock) {
+    for (int q = tid; q < A.ngco * NC; q += kAdjBlock) {
       double sacc = 0.0;
-#pragma unroll
-      for (int rep = 0; rep < kGReps; ++rep) sacc += grep[rep * A.ngco * 6 + q];
-      atab[A.gco0 * 6 + q] += sacc;
+      for (int rep = 0; rep < R; ++rep) sacc += gacc[(size_t)q * R + rep];
+      atab[(A.gco0 + q / NC) * 6 + q % NC] += sacc;
     }
     __syncthreads();
   }
@@ -621,9 +673,22 @@ int setup_adjoint(mz_ctx* c) {
   X.ntab = c->ncoef * 6 + c->nspl * 6;
   X.gco0 = c->meam ? c->coef_off[c->pot.fn_off[4]] : 0;
   X.ngco = c->meam ? c->ncoef - X.gco0 : 0;
-  X.dyn_bytes = c->table_bytes + (X.ntab + (c->meam ? mz::kGReps * X.ngco * 6 : 0)) * (int)sizeof(double);
-  if (X.dyn_bytes + (int)(sizeof(mz::Partner) * mz::kAdjBlock) + 4096 > 64 * 1024)
-    return mz_fail(MZ_ERR_CAPACITY, "adjoint workgroup would exceed 64 KB of LDS");
+  // LDS: spline table, compact g coefficients, coefficient adjoints and the
+  // lane replicas of the g-coefficient adjoints.  The replica count is the
+  // largest power of two that keeps four workgroups per CU (40 KB each), else
+  // the largest that fits the 64 KB workgroup limit.
+  const int nc_g = c->pot.quintic ? 6 : 4;
+  const int fixed = c->table_bytes + (X.ngco * 6 + X.ntab) * (int)sizeof(double);
+  const int stat = (int)(sizeof(mz::Partner) * mz::kAdjBlock + 2 * sizeof(double) * mz::kAdjBlock +
+                         3 * sizeof(double) * mz::kAdjAtoms + sizeof(int) * (mz::kAdjAtoms + 1));
+  X.grep_log2 = -1;
+  for (int lim : {40 * 1024, 64 * 1024}) {
+    for (int lg = 5; lg >= 0 && X.grep_log2 < 0; --lg)
+      if (fixed + (c->meam ? X.ngco * nc_g * (1 << lg) * 8 : 0) + stat <= lim) X.grep_log2 = lg;
+    if (X.grep_log2 >= 0) break;
+  }
+  if (X.grep_log2 < 0) return mz_fail(MZ_ERR_CAPACITY, "adjoint workgroup would exceed 64 KB of LDS");
+  X.dyn_bytes = fixed + (c->meam ? X.ngco * nc_g * (1 << X.grep_log2) * 8 : 0);
   X.ready = true;
   return MZ_OK;
 }
@@ -690,6 +755,7 @@ static int run_grad(mz_ctx* c, int ncand) {
   a.ntab = X.ntab;
   a.gco0 = X.gco0;
   a.ngco = X.ngco;
+  a.grep_log2 = X.grep_log2;
   const bool q = c->pot.quintic != 0;
   if (c->meam) {
     if (c->point) { if (q) launch_adjoint_t<true, true, true>(st, a, X.nchunks, ncand, X.dyn_bytes);

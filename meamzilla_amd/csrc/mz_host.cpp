@@ -315,7 +315,7 @@ int ensure_eval_buffers(mz_ctx* c, int ncand) {
   if (ncand <= c->ncap) return MZ_OK;
   drop_graphs(c);
   const size_t nc = (size_t)ncand;
-  c->resid_blocks = (c->natoms + c->ncells + 255) / 256;
+  c->resid_blocks = mz::resid_blocks(c->natoms, c->ncells);
   if (c->d_knots.alloc(nc * c->nknot) || c->d_tables.alloc(nc * c->table_bytes) ||
       c->d_forces.alloc(nc * c->natoms * 3) || c->d_tvec.alloc(nc * c->npairs * 3) ||
       c->d_fown.alloc(nc * c->natoms * 3) || c->d_chunk_acc.alloc(nc * c->chunks.size() * 8) ||
@@ -389,6 +389,11 @@ mz::ResidArgs resid_args(const mz_ctx* c) {
   r.force0 = c->d_force0.p;
   r.energy_weight = c->energy_weight;
   r.stress_weight = c->stress_weight;
+  r.npairs = c->npairs;
+  r.in_off = c->d_in_off.p;
+  r.in_list = c->d_in_list.p;
+  r.tvec = c->d_tvec.p;
+  r.fown = c->d_fown.p;
   r.forces = c->d_forces.p;
   r.chunk_acc = c->d_chunk_acc.p;
   r.cell_chunk_off = c->d_cell_chunk_off.p;
@@ -442,16 +447,6 @@ int mz_run_forward(mz_ctx* c, int ncand, double* d_errvec, double* d_ce, double*
     return fail(MZ_ERR_ARG, "unsupported tile (block, pairs per lane)");
   MZ_HIP(hipGetLastError());
   if (c->timing) MZ_HIP(hipEventRecord(c->ev[7], st));
-  mz::GatherArgs ga{};
-  ga.natoms = c->natoms;
-  ga.npairs = c->npairs;
-  ga.in_off = c->d_in_off.p;
-  ga.in_list = c->d_in_list.p;
-  ga.tvec = c->d_tvec.p;
-  ga.fown = c->d_fown.p;
-  ga.forces = c->d_forces.p;
-  mz::launch_gather(st, ga, ncand);
-  if (c->timing) MZ_HIP(hipEventRecord(c->ev[3], st));
   mz::ResidArgs ra = resid_args(c);
   ra.errvec = d_errvec;
   ra.cell_energy = d_ce;
@@ -481,7 +476,7 @@ void mz_record_timing(mz_ctx* c) {
   float t01 = 0, t23 = 0, t34 = 0, t04 = 0;
   (void)hipEventElapsedTime(&t01, c->ev[0], c->ev[1]);
   (void)hipEventElapsedTime(&t23, c->ev[2], c->ev[7]);
-  (void)hipEventElapsedTime(&t34, c->ev[3], c->ev[4]);
+  (void)hipEventElapsedTime(&t34, c->ev[7], c->ev[4]);
   (void)hipEventElapsedTime(&t04, c->ev[0], c->ev[4]);
   c->last.refresh_ms = t01;
   c->last.atom_ms = t23;

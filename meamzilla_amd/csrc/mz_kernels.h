@@ -55,7 +55,12 @@ struct ResidArgs {
   const double* stress0;
   const double* force0;
   double energy_weight, stress_weight;
-  const double* forces;
+  long long npairs;
+  const int* in_off;            // [natoms+1] CSR of the pairs whose neighbour is each atom
+  const int* in_list;
+  const double* tvec;           // [ncand][npairs*3] force vector of every pair, on its owner
+  const double* fown;           // [ncand][natoms*3] sum of T over each atom's own pairs
+  double* forces;               // [ncand][natoms*3] F_a = own sum - incoming sum
   const double* chunk_acc;      // [ncand][nchunks][8]
   const int* cell_chunk_off;    // [ncells+1] CSR of the chunks of every cell
   const int* cell_chunks;
@@ -66,17 +71,6 @@ struct ResidArgs {
   double* partial;        // [ncand][nblocks]
   double* lam;            // optional [ncand][natoms*3]: dF/d(force) = 2 w (F - F0)
   double* cell_adj;       // optional [ncand][ncells*8]: dF/dE_raw, dF/dS_raw (6), dF/dP (constraint)
-};
-
-// F_a = (sum of T over a's own pairs) - (sum of T over the pairs pointing at a)
-struct GatherArgs {
-  int natoms;
-  long long npairs;
-  const int* in_off;      // [natoms+1] CSR of the pairs whose neighbour is each atom
-  const int* in_list;
-  const double* tvec;
-  const double* fown;
-  double* forces;         // [ncand][natoms*3]
 };
 
 struct FinalArgs {
@@ -98,7 +92,7 @@ void launch_refresh(hipStream_t st, const double* knots, int nknot, const Refres
                     int table_bytes, int ncand);
 void launch_zero(hipStream_t st, double* p, size_t n);
 int launch_atoms(hipStream_t st, const AtomArgs& a, int nchunks, int ncand, int block, int ppl);
-void launch_gather(hipStream_t st, const GatherArgs& a, int ncand);
+int resid_blocks(int natoms, int ncells);
 void launch_resid(hipStream_t st, const ResidArgs& a, int nblocks, int ncand);
 void launch_final(hipStream_t st, const FinalArgs& a, int ncand);
 

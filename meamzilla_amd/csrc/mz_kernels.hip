@@ -9,16 +9,18 @@
 //               forces and virial (PotMEAMSpline::fast_compute
 //               src/pots/pot_meam_spline/pot_meam_spline.cpp:143-270 and its
 //               EAM / point-evaluated siblings)
-//   k_resid     residual vector and per-block sums of squares
+//   k_resid     force assembly from the stored pair force vectors, residual
+//               vector and per-block sums of squares
 //               (Potential::accumulate_error src/potential.cpp:415-491)
 //   k_final     deterministic final sum plus the style penalty
 //               (pot_meam_spline.cpp:274-278, pot_eam_spline.cpp:211-218)
 //
-// FP64 throughout.  The triplet pass uses a pull formulation: the lane that owns
-// pair (i,j) loops over every other neighbour k of i and accumulates the
-// three-body force on its own leg, so nothing is scattered inside the triplet
-// loop; per-atom sums go through LDS, the neighbour force scatter through
-// global FP64 atomics.
+// FP64 throughout, no global atomics in the forward pass.  The triplet pass is a
+// balanced round-robin over each atom's neighbour pairs: every (j,k) pair of
+// neighbours is visited once by the lane owning j, which keeps its own leg in
+// registers and pushes the partner leg into an LDS accumulator.  Each pair
+// stores its force vector T_p once; an atom's force is its own sum minus the
+// sum over the pairs that point at it, gathered in k_resid in a fixed order.
 #include <hip/hip_runtime.h>
 
 #include "mz_common.h"
@@ -499,84 +501,82 @@ __global__ void __launch_bounds__(BLOCK) k_atoms(const AtomArgs args) {
   }
 }
 
-// ------------------------------------------------------------------ k_gather
-
-// One thread per atom; every pair pointing at the atom carries (minus) the
-// force its owner exerted on it.  Fixed summation order, no atomics.
-__global__ void __launch_bounds__(256) k_gather(const GatherArgs a) {
-  const int cand = blockIdx.y;
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= a.natoms) return;
-  const double* tv = a.tvec + (size_t)cand * a.npairs * 3;
-  const double* fo = a.fown + ((size_t)cand * a.natoms + i) * 3;
-  double fx = fo[0], fy = fo[1], fz = fo[2];
-  const int b = a.in_off[i], e = a.in_off[i + 1];
-  for (int q = b; q < e; ++q) {
-    const double* t = tv + 3 * (size_t)a.in_list[q];
-    fx -= t[0];
-    fy -= t[1];
-    fz -= t[2];
-  }
-  double* f = a.forces + ((size_t)cand * a.natoms + i) * 3;
-  f[0] = fx;
-  f[1] = fy;
-  f[2] = fz;
-}
-
 // ------------------------------------------------------------------ k_resid
+
+// Eight lanes per item; items are the atoms, then the cells.  An atom's lanes
+// gather the force vectors of the pairs pointing at it (F_a = own sum minus
+// incoming sum, fixed order, no atomics) and form its three force residuals;
+// a cell's lanes each reduce one of the eight per-chunk accumulators (energy,
+// six virial components, density constraint) over the cell's chunks.
+// Residuals follow Potential::accumulate_error (src/potential.cpp:415-491).
+constexpr int kResidLanes = 8;
 
 template <int BLOCK>
 __global__ void __launch_bounds__(BLOCK) k_resid(const ResidArgs a) {
   __shared__ double red[BLOCK / 64];
   const int cand = blockIdx.y;
-  const int idx = blockIdx.x * BLOCK + threadIdx.x;
+  const int item = (blockIdx.x * BLOCK + threadIdx.x) / kResidLanes;
+  const int l = threadIdx.x & (kResidLanes - 1);
   double sq = 0.0;
   double* ev = a.errvec ? a.errvec + (size_t)cand * a.nerr : nullptr;
-  const double* cacc_all = a.chunk_acc + (size_t)cand * a.nchunks * 8;
-  if (idx < a.natoms) {
-    const int c = a.cell_of[idx];
-    const double w = a.weight[c];
-    const double* F = a.forces + ((size_t)cand * a.natoms + idx) * 3;
-    const size_t e0 = 7 * (size_t)(c + 1) + 3 * (size_t)idx;
-#pragma unroll
-    for (int m = 0; m < 3; ++m) {
-      const double e = w * (F[m] - a.force0[3 * (size_t)idx + m]);
-      sq += e * e;
-      if (ev) ev[e0 + m] = e;
-      if (a.lam) a.lam[((size_t)cand * a.natoms + idx) * 3 + m] = 2.0 * e * w;
+  if (item < a.natoms) {
+    const int i = item;
+    const double* tv = a.tvec + (size_t)cand * a.npairs * 3;
+    double sx = 0.0, sy = 0.0, sz = 0.0;
+    const int b = a.in_off[i], e = a.in_off[i + 1];
+    for (int q = b + l; q < e; q += kResidLanes) {
+      const double* t = tv + 3 * (size_t)a.in_list[q];
+      sx += t[0];
+      sy += t[1];
+      sz += t[2];
     }
-  } else if (idx < a.natoms + a.ncells) {
-    const int c = idx - a.natoms;
+#pragma unroll
+    for (int o = kResidLanes / 2; o > 0; o >>= 1) {
+      sx += __shfl_xor(sx, o, kResidLanes);
+      sy += __shfl_xor(sy, o, kResidLanes);
+      sz += __shfl_xor(sz, o, kResidLanes);
+    }
+    if (l < 3) {
+      const int c = a.cell_of[i];
+      const double w = a.weight[c];
+      const size_t ai = ((size_t)cand * a.natoms + i) * 3 + l;
+      const double F = a.fown[ai] - (l == 0 ? sx : (l == 1 ? sy : sz));
+      const double er = w * (F - a.force0[3 * (size_t)i + l]);
+      sq = er * er;
+      a.forces[ai] = F;
+      if (ev) ev[7 * (size_t)(c + 1) + 3 * (size_t)i + l] = er;
+      if (a.lam) a.lam[ai] = 2.0 * er * w;
+    }
+  } else if (item < a.natoms + a.ncells) {
+    const int c = item - a.natoms;
+    const double* cacc = a.chunk_acc + (size_t)cand * a.nchunks * 8;
+    double acc = 0.0;
+    for (int q = a.cell_chunk_off[c]; q < a.cell_chunk_off[c + 1]; ++q) acc += cacc[8 * (size_t)a.cell_chunks[q] + l];
     const int n = a.natoms_c[c];
-    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (int q = a.cell_chunk_off[c]; q < a.cell_chunk_off[c + 1]; ++q) {
-      const double* cp = cacc_all + 8 * (size_t)a.cell_chunks[q];
-#pragma unroll
-      for (int m = 0; m < 8; ++m) acc[m] += cp[m];
-    }
-    const double E = acc[0] / n;
     const size_t e0 = 7 * (size_t)c + 3 * (size_t)a.abase[c];
-    const double WE = a.weight[c] * a.eweight[c] * a.energy_weight;
-    const double eE = WE * n * (E - a.energy0[c]);
-    sq += eE * eE;
     double* adj = a.cell_adj ? a.cell_adj + ((size_t)cand * a.ncells + c) * 8 : nullptr;
-    if (adj) adj[0] = 2.0 * eE * WE;
-    if (ev) ev[e0] = eE;
-    if (a.cell_energy) a.cell_energy[(size_t)cand * a.ncells + c] = E;
-#pragma unroll
-    for (int m = 0; m < 6; ++m) {
-      const double S = acc[1 + m] / a.vol[c];
+    double er, dadj;
+    if (l == 0) {
+      const double E = acc / n;
+      const double WE = a.weight[c] * a.eweight[c] * a.energy_weight;
+      er = WE * n * (E - a.energy0[c]);
+      dadj = 2.0 * er * WE;
+      if (ev) ev[e0] = er;
+      if (a.cell_energy) a.cell_energy[(size_t)cand * a.ncells + c] = E;
+    } else if (l < 7) {
+      const double S = acc / a.vol[c];
       const double WS = a.weight[c] * a.sweight[c] * a.stress_weight;
-      const double es = WS * (S - a.stress0[6 * c + m]);
-      sq += es * es;
-      if (adj) adj[1 + m] = 2.0 * es * WS / a.vol[c];
-      if (ev) ev[e0 + 1 + m] = es;
-      if (a.cell_stress) a.cell_stress[((size_t)cand * a.ncells + c) * 6 + m] = S;
+      er = WS * (S - a.stress0[6 * c + l - 1]);
+      dadj = 2.0 * er * WS / a.vol[c];
+      if (ev) ev[e0 + l] = er;
+      if (a.cell_stress) a.cell_stress[((size_t)cand * a.ncells + c) * 6 + l - 1] = S;
+    } else {
+      er = acc;
+      dadj = 2.0 * er;
+      if (ev) ev[7 * (size_t)a.ncells + 3 * (size_t)a.natoms + c] = er;
     }
-    const double con = acc[7];
-    sq += con * con;
-    if (adj) adj[7] = 2.0 * con;
-    if (ev) ev[7 * (size_t)a.ncells + 3 * (size_t)a.natoms + c] = con;
+    sq = er * er;
+    if (adj) adj[l] = dadj;
   }
   const double t = block_sum<BLOCK>(sq, red);
   if (threadIdx.x == 0) a.partial[(size_t)cand * gridDim.x + blockIdx.x] = t;
@@ -672,8 +672,8 @@ int launch_atoms(hipStream_t st, const AtomArgs& a, int nchunks, int ncand, int 
   return 0;
 }
 
-void launch_gather(hipStream_t st, const GatherArgs& a, int ncand) {
-  hipLaunchKernelGGL(k_gather, dim3((a.natoms + 255) / 256, ncand), dim3(256), 0, st, a);
+int resid_blocks(int natoms, int ncells) {
+  return (int)(((long long)(natoms + ncells) * kResidLanes + 255) / 256);
 }
 
 void launch_resid(hipStream_t st, const ResidArgs& a, int nblocks, int ncand) {
