@@ -722,6 +722,7 @@ void launch_adjoint_t(hipStream_t st, const mz::AdjArgs& a, int nchunks, int nca
 int mz_run_forward(mz_ctx* c, int ncand, double* d_errvec, double* d_ce, double* d_cs, double* d_rho, double* d_dF,
                    double* d_G, double* d_lam, double* d_cadj);
 mz::AtomArgs mz_atom_args(const mz_ctx* c);
+void mz_record_timing(mz_ctx* c);
 int mz_ensure_buffers(mz_ctx* c, int ncand);
 
 // Launch sequence of objective + gradient for ncand staged knot vectors.
@@ -828,6 +829,7 @@ int mz_grad_impl(mz_ctx* c, int ncand, const double* knots, double* F, double* G
       ADJ_HIP(hipStreamSynchronize(st));
       float ms = 0;
       (void)hipEventElapsedTime(&ms, c->ev[5], c->ev[6]);
+      mz_record_timing(c);
       c->last.adjoint_ms = ms;
     } else {
       hipGraphExec_t ex = nullptr;

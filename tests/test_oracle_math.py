@@ -168,3 +168,59 @@ def test_objective_is_sum_of_squared_residuals():
     n = w.db.natoms[0]
     assert abs(r["errvec"][0] - n * (r["cell_energy"][0] - w.db.energy0[0])) < 1e-12 * abs(r["errvec"][0]) + 1e-15
     assert abs(r["errvec"][7] - (r["forces"][0, 0] - w.db.force0[0, 0])) < 1e-13
+
+
+def _brute_counts(db, rc, reach=4):
+    """Full pair list by a fixed, generous image range (no pruning), and
+    sum over atoms of C(n_i, 2)."""
+    npair, ntrip, a0 = 0, 0, 0
+    for c in range(db.ncells):
+        n = int(db.natoms[c])
+        pos = db.pos[a0:a0 + n]
+        box = db.box[c]
+        img = np.array([[i, j, k] for i in range(-reach, reach + 1) for j in range(-reach, reach + 1)
+                        for k in range(-reach, reach + 1)], float) @ box
+        for i in range(n):
+            d = (pos[None, :, :] + img[:, None, :] - pos[i][None, None, :]).reshape(-1, 3)
+            r = np.sqrt((d * d).sum(1))
+            ni = int(((r < rc) & (r > 0)).sum())
+            npair += ni
+            ntrip += ni * (ni - 1) // 2
+        a0 += n
+    return npair, ntrip
+
+
+@pytest.mark.parametrize("cfg", [0, 1, 2])
+def test_pair_and_triplet_counts_match_brute_force(cfg):
+    w = gen.workload(cfg, scale={0: 4 / 16, 1: 4 / 256, 2: 6 / 512}[cfg])
+    o = pyoracle.Oracle(w.pot, w.db)
+    npair, ntrip = _brute_counts(w.db, w.pot.max_rcut())
+    meam = w.pot.style.startswith("meam")
+    assert o.counts() == (npair, ntrip if meam else 0)
+
+
+def test_objective_invariant_under_translation_and_permutation():
+    w = gen.workload(1, scale=4 / 256)
+    gen.add_targets(w, lambda pot, db: _cells(pyoracle.Oracle(pot, db).eval()))
+    base = pyoracle.Oracle(w.pot, w.db).eval()
+    db = w.db.subset(range(w.db.ncells))
+    db.pos = db.pos + np.array([0.37, -1.21, 2.05])
+    moved = pyoracle.Oracle(w.pot, db).eval()
+    assert abs(moved["F"] - base["F"]) <= 1e-10 * abs(base["F"])
+    assert np.allclose(moved["forces"], base["forces"], rtol=0, atol=1e-10)
+    # permute the atoms of every cell (targets permuted with them)
+    db = w.db.subset(range(w.db.ncells))
+    perm, a0 = [], 0
+    for n in db.natoms:
+        perm.extend(a0 + RNG.permutation(int(n)))
+        a0 += int(n)
+    perm = np.array(perm)
+    db.pos, db.types, db.force0 = db.pos[perm], db.types[perm], db.force0[perm]
+    shuf = pyoracle.Oracle(w.pot, db).eval()
+    assert abs(shuf["F"] - base["F"]) <= 1e-10 * abs(base["F"])
+    assert np.allclose(shuf["forces"], np.asarray(base["forces"]).reshape(-1, 3)[perm].reshape(
+        np.shape(shuf["forces"])), rtol=0, atol=1e-10)
+
+
+def _cells(r):
+    return r["cell_energy"], r["cell_stress"], r["forces"]
