@@ -61,7 +61,12 @@ struct PotDesc {
   int32_t quintic;      // any spline5eq present
   int32_t gco0;         // first interval record of the g splines (MEAM)
   int32_t ngco;         // interval records of the g splines
+  int32_t g_keep_all;   // every g spline's x_n (1 + 1e-8) exceeds any cosine: no triplet is cut
 };
+
+// Doubles per g-interval record in the atom kernel's compact LDS copy
+// (c0..c3 for cubic, c0..c5 padded to 8 for quintic splines).
+constexpr int g_record_doubles(bool quintic) { return quintic ? 8 : 4; }
 
 // Packs an image triple into one 32-bit word (8 bits each, signed).
 inline uint32_t pack_image(int ix, int iy, int iz) {

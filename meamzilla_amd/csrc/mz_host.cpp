@@ -101,6 +101,17 @@ int build_layout(mz_ctx* c, const mz_layout* L) {
   c->pot.quintic = quintic;
   c->pot.gco0 = c->meam ? c->coef_off[c->pot.fn_off[4]] : coff;
   c->pot.ngco = c->meam ? coff - c->pot.gco0 : 0;
+  // |cos| <= 1 + a few ulps, so a knot-style g spline ending at or beyond 1
+  // never cuts a triplet
+  c->pot.g_keep_all = 0;
+  if (c->meam && !c->point) {
+    bool all = true;
+    for (int t = 0; t < nt; ++t) {
+      const int s = c->pot.fn_off[4] + t;
+      if (!(c->xn[s] * (1. + 1.e-8) > 1.0 + 1.e-12)) all = false;
+    }
+    c->pot.g_keep_all = all ? 1 : 0;
+  }
   return MZ_OK;
 }
 
@@ -442,8 +453,13 @@ int mz_run_forward(mz_ctx* c, int ncand, double* d_errvec, double* d_ce, double*
   aa.rho_out = d_rho;
   aa.dF_out = d_dF;
   aa.G_out = d_G;
+  aa.ncand = ncand;
   const int ppl = std::max(1, c->tile.max_pairs / c->tile.block);
-  if (mz::launch_atoms(st, aa, (int)c->chunks.size(), ncand, c->tile.block, ppl))
+  // candidates per workgroup: the tile's choice, at most what ncand fills and
+  // what the LDS holds
+  int ct = std::min(c->tile.cand_tile, ncand >= 4 ? 4 : (ncand >= 2 ? 2 : 1));
+  while (ct > 1 && mz::atoms_static_lds(c->tile.block, ppl, ct) + mz::atoms_dyn_lds(c->pot, ct) > 64 * 1024) ct >>= 1;
+  if (mz::launch_atoms(st, aa, (int)c->chunks.size(), ncand, c->tile.block, ppl, ct))
     return fail(MZ_ERR_ARG, "unsupported tile (block, pairs per lane)");
   MZ_HIP(hipGetLastError());
   if (c->timing) MZ_HIP(hipEventRecord(c->ev[7], st));
@@ -781,10 +797,10 @@ int mz_set_tile(mz_ctx* c, const mz_tile* t) {
   const bool ok = (t->block == 256 && (ppl == 1 || ppl == 2)) || (t->block == 128 && (ppl == 2 || ppl == 4)) ||
                   (t->block == 512 && ppl == 1) || (t->block == 64 && ppl == 4);
   if (!ok || ppl * t->block != t->max_pairs) return fail(MZ_ERR_ARG, "unsupported tile (block, max_pairs)");
-  // static LDS of the atom kernel: 72 bytes per pair slot plus ~3.5 KB, and
-  // the staged spline tables
-  const size_t lds = (size_t)t->max_pairs * 72 + 3584 + c->table_bytes + (size_t)c->pot.ngco * 48;
-  if (lds > 64 * 1024) return fail(MZ_ERR_CAPACITY, "tile and spline tables exceed 64 KB of LDS");
+  if (t->cand_tile != 0 && t->cand_tile != 1 && t->cand_tile != 2 && t->cand_tile != 4)
+    return fail(MZ_ERR_ARG, "cand_tile must be 1, 2 or 4");
+  if (mz::atoms_static_lds(t->block, ppl, 1) + mz::atoms_dyn_lds(c->pot, 1) > 64 * 1024)
+    return fail(MZ_ERR_CAPACITY, "tile and spline tables exceed 64 KB of LDS");
   mz_tile old = c->tile;
   drop_graphs(c);
   c->tile = *t;

@@ -30,6 +30,7 @@ struct AtomArgs {
   const double* weight;   // [ncells]
   const char* tables;     // [ncand][table_bytes]
   int table_bytes;
+  int ncand;              // candidates of this launch
   int natoms;
   int ncells;
   long long npairs;
@@ -91,7 +92,11 @@ struct FinalArgs {
 void launch_refresh(hipStream_t st, const double* knots, int nknot, const RefreshMeta& meta, char* tables,
                     int table_bytes, int ncand);
 void launch_zero(hipStream_t st, double* p, size_t n);
-int launch_atoms(hipStream_t st, const AtomArgs& a, int nchunks, int ncand, int block, int ppl);
+// CT candidates per workgroup (1, 2 or 4); returns -1 for an unsupported shape.
+int launch_atoms(hipStream_t st, const AtomArgs& a, int nchunks, int ncand, int block, int ppl, int ct);
+// Dynamic (spline tables) and static LDS bytes of one atom workgroup.
+size_t atoms_dyn_lds(const PotDesc& pot, int ct);
+size_t atoms_static_lds(int block, int ppl, int ct);
 int resid_blocks(int natoms, int ncells);
 void launch_resid(hipStream_t st, const ResidArgs& a, int nblocks, int ncand);
 void launch_final(hipStream_t st, const FinalArgs& a, int ncand);

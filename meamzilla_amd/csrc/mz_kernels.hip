@@ -214,64 +214,237 @@ __global__ void k_zero(double* __restrict__ p, size_t n) {
 
 // ------------------------------------------------------------------ k_atoms
 
-// LDS layout of one workgroup (static part).
-template <int BLOCK, int PPL, int MAXA>
-struct AtomSmem {
-  double kd[BLOCK * PPL][6];   // per pair: unit vector, f, f', 1/r (48-byte records)
-  double gl[3][BLOCK * PPL];   // three-body leg sums pushed by the partner lanes
-  double rho[MAXA];
-  double dF[MAXA];
-  double fi[MAXA][3];
-  int poff[MAXA + 1];
-  double red8[8][BLOCK / 64];
+// Location of r on a radial spline, shared by every candidate of a workgroup
+// (the knots of all candidates sit at the same x).  reg: 0 interval k with
+// shift t, 1 left of x_0 (point style, t = r - x_0).  Returns false when the
+// function is cut off: knot style beyond x_n (1 + 1e-8) (src/spline.cpp:221-246),
+// point style beyond x_n.
+template <bool POINT>
+__device__ __forceinline__ bool locate_radial(const SplHead& H, const SplInterval* IV, double r, int& k, double& t,
+                                              int& reg) {
+  reg = 0;
+  if (POINT) {
+    if (!(r <= H.xn)) return false;
+    if (r < H.x0) {
+      reg = 1;
+      t = r - H.x0;
+      k = 0;
+      return true;
+    }
+    const double delta = r - H.x0;
+    k = (int)__dmul_rn(delta, H.invstep);
+    if (k >= H.n - 1) k = H.n - 2;
+    if (k < 0) k = 0;
+    t = __dmul_rn(__dadd_rn(delta, -__dmul_rn((double)k, H.step)), H.invstep);
+    return true;
+  }
+  if (!(r < H.xcut)) return false;
+  const SplInterval* I = IV + H.coef_off;
+  const int nm2 = H.n - 2;
+  k = (int)floor((r - H.x0) * H.invstep);
+  k = k < 0 ? 0 : (k > nm2 ? nm2 : k);
+  while (k > 0 && I[k].xk > r) --k;
+  while (k < nm2 && I[k + 1].xk <= r) ++k;
+  t = (r - I[k].xk) * I[k].invh;
+  return true;
+}
+
+// Value and d/dr of one candidate's spline at a located r.
+template <bool Q>
+__device__ __forceinline__ void eval_located(const SplHead& H, const SplInterval* IV, int k, double t, int reg,
+                                             double& v, double& d) {
+  if (reg == 1) {
+    d = H.gL + H.g2L * t;
+    v = H.y0 + H.gL * t + 0.5 * H.g2L * t * t;
+    return;
+  }
+  poly<Q>(IV[H.coef_off + k], t, H.invstep, v, d);
+}
+
+// Shared part of a g evaluation: interval and shift of cos (knot positions are
+// common to all candidates), or the extrapolation side (point style).
+struct GLoc {
+  int off;    // record offset (doubles) of the interval in a compact g table
+  int side;   // 0 inside, 1 left of x_0, 2 right of x_n (point style)
+  double t;   // shift in the interval, or cos - end when extrapolating
 };
 
-template <int BLOCK, int PPL, int MAXA, bool MEAM, bool POINT, bool Q>
+template <bool POINT, bool Q>
+__device__ __forceinline__ GLoc g_locate(const SplHead& Hg, double cs) {
+  GLoc L;
+  if (POINT && (cs < Hg.x0 || cs > Hg.xn)) {
+    L.side = cs < Hg.x0 ? 1 : 2;
+    L.t = cs - (L.side == 1 ? Hg.x0 : Hg.xn);
+    L.off = 0;
+    return L;
+  }
+  const double u = (cs - Hg.x0) * Hg.invstep;
+  const double fl = fmin(fmax(floor(u), 0.0), (double)(Hg.n - 2));
+  L.t = u - fl;
+  L.off = g_record_doubles(Q) * (int)fl;
+  L.side = 0;
+  return L;
+}
+
+// g and dg/dcos of one candidate at a located cosine.
+template <bool POINT, bool Q>
+__device__ __forceinline__ void g_value(const SplHead& Hg, const double* gtg, const GLoc& L, double& gv, double& gp) {
+  if (POINT && L.side != 0) {
+    const bool left = L.side == 1;
+    const double e = L.t;
+    const double g1 = left ? Hg.gL : Hg.gR, g2 = left ? Hg.g2L : Hg.g2R;
+    gv = (left ? Hg.y0 : Hg.yn) + g1 * e + 0.5 * g2 * e * e;
+    gp = g1 + g2 * e;
+    return;
+  }
+  const double t = L.t;
+  const double2* C = reinterpret_cast<const double2*>(gtg + L.off);
+  const double2 c01 = C[0], c23 = C[1];
+  double dgt;
+  if (Q) {
+    const double2 c45 = C[2];
+    gv = c01.x + t * (c01.y + t * (c23.x + t * (c23.y + t * (c45.x + t * c45.y))));
+    dgt = c01.y + t * (2.0 * c23.x + t * (3.0 * c23.y + t * (4.0 * c45.x + t * 5.0 * c45.y)));
+  } else {
+    gv = c01.x + t * (c01.y + t * (c23.x + t * c23.y));
+    dgt = c01.y + t * (2.0 * c23.x + t * 3.0 * c23.y);
+  }
+  gp = dgt * Hg.invstep;
+}
+
+// LDS layout of one workgroup (static part).  CT candidates share the pair
+// geometry; f of candidate 0 rides in the partner record, the others in fx.
+template <int BLOCK, int PPL, int MAXA, int CT>
+struct AtomSmem {
+  double4 rec[BLOCK * PPL];                 // per pair: unit vector, f (candidate 0); later T
+  double fx[CT > 1 ? CT - 1 : 1][BLOCK * PPL];  // f of candidates 1..CT-1
+  double rp[CT][BLOCK * PPL];               // per pair: its density contribution to the owning atom
+  double rho[CT][MAXA];
+  double dF[CT][MAXA];
+  int poff[MAXA + 1];
+  double red8[CT][8][BLOCK / 64];
+};
+
+// Sums three per-pair values (vals[q] for q in the atom's pair range) for
+// every atom of the chunk, eight lanes per atom, fixed order; out(a, x, y, z)
+// receives the sums.  Every lane of the workgroup must call it.
+template <int BLOCK, class Get, class Out>
+__device__ __forceinline__ void atom_sums3(const int* poff, int na, Get get, Out out) {
+  const int l = threadIdx.x & 7;
+  for (int base = 0; base < na; base += BLOCK / 8) {
+    const int a = base + (int)(threadIdx.x >> 3);
+    double x = 0.0, y = 0.0, z = 0.0;
+    if (a < na) {
+      for (int q = poff[a] + l; q < poff[a + 1]; q += 8) {
+        double vx, vy, vz;
+        get(q, vx, vy, vz);
+        x += vx;
+        y += vy;
+        z += vz;
+      }
+    }
+#pragma unroll
+    for (int o = 4; o > 0; o >>= 1) {
+      x += __shfl_xor(x, o, 8);
+      y += __shfl_xor(y, o, 8);
+      z += __shfl_xor(z, o, 8);
+    }
+    if (a < na && l == 0) out(a, x, y, z);
+  }
+}
+
+// Wave totals of eight per-lane values with ten shuffles: three halving
+// exchanges leave each lane with one value index summed over eight lanes, three
+// more steps finish the sum.  Lane l with (l & 7) == 0 returns the total of
+// value index ((l >> 5) & 1) * 4 + ((l >> 4) & 1) * 2 + ((l >> 3) & 1).
+__device__ __forceinline__ double wave_sums8(const double (&v)[8]) {
+  const int lane = threadIdx.x & 63;
+  double a[4], b[2];
+  const bool h5 = lane & 32, h4 = lane & 16, h3 = lane & 8;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const double send = h5 ? v[i] : v[i + 4];
+    const double keep = h5 ? v[i + 4] : v[i];
+    a[i] = keep + __shfl_xor(send, 32, 64);
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const double send = h4 ? a[i] : a[i + 2];
+    const double keep = h4 ? a[i + 2] : a[i];
+    b[i] = keep + __shfl_xor(send, 16, 64);
+  }
+  double c = (h3 ? b[1] : b[0]) + __shfl_xor(h3 ? b[0] : b[1], 8, 64);
+  c += __shfl_xor(c, 4, 64);
+  c += __shfl_xor(c, 2, 64);
+  c += __shfl_xor(c, 1, 64);
+  return c;
+}
+
+// One workgroup per (chunk, group of CT candidates).
+template <int BLOCK, int PPL, int MAXA, int CT, bool MEAM, bool POINT, bool Q>
 __global__ void __launch_bounds__(BLOCK) k_atoms(const AtomArgs args) {
   extern __shared__ __attribute__((aligned(64))) char dyn[];
-  __shared__ __attribute__((aligned(16))) AtomSmem<BLOCK, PPL, MAXA> sm;
-  const int cand = blockIdx.y;
+  __shared__ __attribute__((aligned(16))) AtomSmem<BLOCK, PPL, MAXA, CT> sm;
+  constexpr int NG = g_record_doubles(Q);
   const Chunk ch = args.chunks[blockIdx.x];
   const int tid = threadIdx.x;
+  const int TB = args.table_bytes;
+  int cand[CT];
+  bool valid[CT];
+#pragma unroll
+  for (int c = 0; c < CT; ++c) {
+    const int cc = blockIdx.y * CT + c;
+    valid[c] = cc < args.ncand;
+    cand[c] = valid[c] ? cc : args.ncand - 1;
+  }
 
-  // stage this candidate's spline table in LDS, plus the g intervals again as
-  // 48-byte records (6 coefficients) so that lanes reading different
-  // intervals hit different banks
-  {
-    const int4* src = reinterpret_cast<const int4*>(args.tables + (size_t)cand * args.table_bytes);
-    int4* dst = reinterpret_cast<int4*>(dyn);
-    for (int q = tid; q < args.table_bytes / 16; q += BLOCK) dst[q] = src[q];
+  // stage the candidates' spline tables in LDS, then their g intervals again
+  // as compact power-of-two records (c0..c3, or c0..c5 padded to 8 doubles)
+  const int gbytes = MEAM ? args.pot.ngco * NG * 8 : 0;
+#pragma unroll
+  for (int c = 0; c < CT; ++c) {
+    const int4* src = reinterpret_cast<const int4*>(args.tables + (size_t)cand[c] * TB);
+    int4* dst = reinterpret_cast<int4*>(dyn + (size_t)c * TB);
+    for (int q = tid; q < TB / 16; q += BLOCK) dst[q] = src[q];
+    if (MEAM) {
+      const SplInterval* gsrc = reinterpret_cast<const SplInterval*>(args.tables + (size_t)cand[c] * TB +
+                                                                     sizeof(SplHead) * args.pot.nspl);
+      double* gdst = reinterpret_cast<double*>(dyn + (size_t)CT * TB + (size_t)c * gbytes);
+      for (int q = tid; q < args.pot.ngco * NG; q += BLOCK)
+        gdst[q] = (q % NG) < 6 ? gsrc[args.pot.gco0 + q / NG].c[q % NG] : 0.0;
+    }
   }
-  const SplHead* HD = reinterpret_cast<const SplHead*>(dyn);
-  const SplInterval* IV = reinterpret_cast<const SplInterval*>(dyn + sizeof(SplHead) * args.pot.nspl);
-  double* gt = reinterpret_cast<double*>(dyn + args.table_bytes);
-  if (MEAM) {
-    const SplInterval* src = reinterpret_cast<const SplInterval*>(args.tables + (size_t)cand * args.table_bytes +
-                                                                  sizeof(SplHead) * args.pot.nspl);
-    for (int q = tid; q < args.pot.ngco * 6; q += BLOCK) gt[q] = src[args.pot.gco0 + q / 6].c[q % 6];
-  }
+  auto HDc = [&](int c) { return reinterpret_cast<const SplHead*>(dyn + (size_t)c * TB); };
+  auto IVc = [&](int c) {
+    return reinterpret_cast<const SplInterval*>(dyn + (size_t)c * TB + sizeof(SplHead) * args.pot.nspl);
+  };
+  auto GTc = [&](int c) { return reinterpret_cast<const double*>(dyn + (size_t)CT * TB + (size_t)c * gbytes); };
   for (int a = tid; a <= ch.na; a += BLOCK) sm.poff[a] = args.atom_poff[ch.a0 + a] - ch.p0;
-  for (int a = tid; a < ch.na; a += BLOCK) {
-    sm.rho[a] = 0.0;
-    sm.fi[a][0] = sm.fi[a][1] = sm.fi[a][2] = 0.0;
-  }
   __syncthreads();
 
   const int nt = args.pot.nt;
   const double* box = args.box + 9 * ch.cell;
+  const SplHead* HD0 = HDc(0);
+  const SplInterval* IV0 = IVc(0);
 
   // per-lane pair state
-  int ia[PPL], jg[PPL];
+  int ia[PPL];
   double r[PPL], invr[PPL], ux[PPL], uy[PPL], uz[PPL];
-  double aphi[PPL], brho[PPL], fv[PPL], dfv[PPL], Gx[PPL], Gy[PPL], Gz[PPL];
-  double e_acc = 0.0;
+  double aphi[PPL][CT], brho[PPL][CT], fv[PPL][CT], dfv[PPL][CT], Gx[PPL][CT], Gy[PPL][CT], Gz[PPL][CT];
+  double e_acc[CT];
+#pragma unroll
+  for (int c = 0; c < CT; ++c) e_acc[c] = 0.0;
 
   // ---- phase 1: geometry and radial functions
 #pragma unroll
   for (int s = 0; s < PPL; ++s) {
     const int p = tid + s * BLOCK;
     ia[s] = -1;
-    Gx[s] = Gy[s] = Gz[s] = 0.0;
+#pragma unroll
+    for (int c = 0; c < CT; ++c) {
+      aphi[s][c] = brho[s][c] = fv[s][c] = dfv[s][c] = 0.0;
+      Gx[s][c] = Gy[s][c] = Gz[s][c] = 0.0;
+    }
     if (p < ch.np) {
       int lo = 0, hi = ch.na - 1;
       while (lo < hi) {  // last atom with poff <= p
@@ -282,7 +455,6 @@ __global__ void __launch_bounds__(BLOCK) k_atoms(const AtomArgs args) {
       const int ig = ch.a0 + lo;
       const uint2 pr = args.pairs[ch.p0 + p];
       const int j = (int)pr.x;
-      jg[s] = j;
       const int ix = (int)(int8_t)(pr.y & 0xff), iy = (int)(int8_t)((pr.y >> 8) & 0xff),
                 iz = (int)(int8_t)((pr.y >> 16) & 0xff);
       const double* pj = args.pos + 3 * (size_t)j;
@@ -305,199 +477,206 @@ __global__ void __launch_bounds__(BLOCK) k_atoms(const AtomArgs args) {
       uz[s] = d[2] / rr;
       const int ti = args.types[ig], tj = args.types[j];
       const int pij = pair_ij(ti, tj, nt);
-      double v, dv;
-      const SplHead& Hphi = HD[args.pot.fn_off[0] + pij];
-      aphi[s] = 0.0;
-      if (eval_radial<POINT, Q>(Hphi, IV, rr, v, dv)) {
-        e_acc += 0.5 * v;
-        aphi[s] = 0.5 * dv;
-      }
-      const SplHead& Hrho = HD[args.pot.fn_off[1] + tj];
-      brho[s] = 0.0;
-      if (eval_radial<POINT, Q>(Hrho, IV, rr, v, dv)) {
-        atomicAdd(&sm.rho[lo], v);
-        brho[s] = dv;
-      }
-      fv[s] = 0.0;
-      dfv[s] = 0.0;
-      if (MEAM) {
-        const SplHead& Hf = HD[args.pot.fn_off[3] + pij];
-        if (eval_radial<POINT, Q>(Hf, IV, rr, v, dv)) {
-          fv[s] = v;
-          dfv[s] = dv;
+      int k, reg;
+      double t;
+      const int sphi = args.pot.fn_off[0] + pij, srho = args.pot.fn_off[1] + tj;
+      if (locate_radial<POINT>(HD0[sphi], IV0, rr, k, t, reg)) {
+#pragma unroll
+        for (int c = 0; c < CT; ++c) {
+          double v, dv;
+          eval_located<Q>(HDc(c)[sphi], IVc(c), k, t, reg, v, dv);
+          e_acc[c] += 0.5 * v;
+          aphi[s][c] = 0.5 * dv;
         }
-        double2* K = reinterpret_cast<double2*>(sm.kd[p]);
-        K[0] = make_double2(ux[s], uy[s]);
-        K[1] = make_double2(uz[s], fv[s]);
-        K[2] = make_double2(dfv[s], invr[s]);
-        sm.gl[0][p] = sm.gl[1][p] = sm.gl[2][p] = 0.0;
+      }
+      double rv[CT];
+#pragma unroll
+      for (int c = 0; c < CT; ++c) rv[c] = 0.0;
+      if (locate_radial<POINT>(HD0[srho], IV0, rr, k, t, reg)) {
+#pragma unroll
+        for (int c = 0; c < CT; ++c) eval_located<Q>(HDc(c)[srho], IVc(c), k, t, reg, rv[c], brho[s][c]);
+      }
+#pragma unroll
+      for (int c = 0; c < CT; ++c) sm.rp[c][p] = rv[c];
+      if (MEAM) {
+        const int sf = args.pot.fn_off[3] + pij;
+        if (locate_radial<POINT>(HD0[sf], IV0, rr, k, t, reg)) {
+#pragma unroll
+          for (int c = 0; c < CT; ++c) eval_located<Q>(HDc(c)[sf], IVc(c), k, t, reg, fv[s][c], dfv[s][c]);
+        }
+        sm.rec[p] = make_double4(ux[s], uy[s], uz[s], fv[s][0]);
+#pragma unroll
+        for (int c = 1; c < CT; ++c) sm.fx[c - 1][p] = fv[s][c];
       }
     }
   }
   __syncthreads();
 
-  // ---- phase 2: triplets, each unordered (j, k) once (dF factored out)
-  // Lane j of an atom with n pairs visits partners k = j+1 .. j+(n-1)/2
-  // (mod n), plus k = j+n/2 for the first n/2 lanes when n is even: every
-  // lane gets the same number of partners.  The own leg accumulates in
-  // registers, the partner leg is pushed into LDS (distinct k per lane and
-  // step, so the atomics never collide).
+  // ---- phase 2: triplets (dF factored out)
+  // With g = g(cos), g' = dg/dcos, every pair j needs, over the other pairs k
+  // of its atom, S1 = sum g f_k, S2 = sum g' f_k cos, S3 = sum g' f_k d_k; its
+  // three-body leg is then G_j = d_j (f'_j S1 - f_j S2 / r_j) + (f_j / r_j) S3
+  // and the atom's density gains f_j S1 / 2 (each unordered triplet is seen
+  // from both of its pairs).  All lanes of an atom walk its pairs k in the same
+  // order, so partner records are LDS broadcasts and nothing is written until
+  // the walk ends; cos and the g interval are shared by the CT candidates.
   if (MEAM) {
+    const bool cut = !POINT && !args.pot.g_keep_all;
 #pragma unroll
     for (int s = 0; s < PPL; ++s) {
       const int p = tid + s * BLOCK;
       if (ia[s] < 0) continue;
       const int kb = sm.poff[ia[s]], ke = sm.poff[ia[s] + 1];
-      const int n = ke - kb, l = p - kb;
-      const int trips = ((n - 1) >> 1) + (((n & 1) == 0 && l < (n >> 1)) ? 1 : 0);
-      const int ti = args.types[ch.a0 + ia[s]];
-      const SplHead& Hg = HD[args.pot.fn_off[4] + ti];
-      const double gx0 = Hg.x0, ginv = Hg.invstep, gxcut = Hg.xcut, gxn = Hg.xn;
-      const double gkmax = (double)(Hg.n - 2);
-      const double gy0 = Hg.y0, gyn = Hg.yn, ggL = Hg.gL, ggR = Hg.gR, gg2L = Hg.g2L, gg2R = Hg.g2R;
-      const double* gtg = gt + (size_t)(Hg.coef_off - args.pot.gco0) * 6;
-      const double fj = fv[s], dfj = dfv[s], ivj = invr[s];
+      const SplHead& Hg = HD0[args.pot.fn_off[4] + args.types[ch.a0 + ia[s]]];
+      const int goff = (Hg.coef_off - args.pot.gco0) * NG;
       const double xj = ux[s], yj = uy[s], zj = uz[s];
-      double rho_t = 0.0, gx = 0.0, gy = 0.0, gz = 0.0;
-      int kl = l;
-      for (int m = 0; m < trips; ++m) {
-        kl = (kl + 1 == n) ? 0 : kl + 1;
-        const int k = kb + kl;
-        const double2* K = reinterpret_cast<const double2*>(sm.kd[k]);
-        const double2 ka = K[0], kbv = K[1], kc = K[2];
-        const double xk = ka.x, yk = ka.y, zk = kbv.x;
-        const double cs = xj * xk + yj * yk + zj * zk;
-        double gv, dgt;  // g and dg/dt (t = shift within the interval)
-        bool keep = true;
-        if (POINT && (cs < gx0 || cs > gxn)) {
-          const bool left = cs < gx0;
-          const double e = cs - (left ? gx0 : gxn);
-          const double g1 = left ? ggL : ggR, g2 = left ? gg2L : gg2R;
-          gv = (left ? gy0 : gyn) + g1 * e + 0.5 * g2 * e * e;
-          dgt = (g1 + g2 * e) / ginv;
-        } else {
-          const double u = (cs - gx0) * ginv;
-          const double fl = fmin(fmax(floor(u), 0.0), gkmax);
-          const double t = u - fl;
-          const double2* C = reinterpret_cast<const double2*>(gtg + 6 * (int)fl);
-          const double2 c01 = C[0], c23 = C[1];
-          if (Q) {
-            const double2 c45 = C[2];
-            gv = c01.x + t * (c01.y + t * (c23.x + t * (c23.y + t * (c45.x + t * c45.y))));
-            dgt = c01.y + t * (2.0 * c23.x + t * (3.0 * c23.y + t * (4.0 * c45.x + t * 5.0 * c45.y)));
-          } else {
-            gv = c01.x + t * (c01.y + t * (c23.x + t * c23.y));
-            dgt = c01.y + t * (2.0 * c23.x + t * 3.0 * c23.y);
-          }
-          if (!POINT) keep = cs < gxcut;
-        }
-        const double fk = keep ? kbv.y : 0.0;
-        const double dfk = keep ? kc.x : 0.0;
-        const double ivk = kc.y;
-        const double ff = fj * fk;
-        rho_t += ff * gv;
-        const double V3 = dgt * ginv * ff;          // V3 / dF
-        const double vlj = V3 * ivj, vlk = V3 * ivk;
-        const double vvj = gv * dfj * fk - vlj * cs;  // (dV3j - vlj cos) / dF
-        const double vvk = gv * fj * dfk - vlk * cs;  // (dV3k - vlk cos) / dF
-        gx += xj * vvj + xk * vlj;
-        gy += yj * vvj + yk * vlj;
-        gz += zj * vvj + zk * vlj;
-        atomicAdd(&sm.gl[0][k], xk * vvk + xj * vlk);
-        atomicAdd(&sm.gl[1][k], yk * vvk + yj * vlk);
-        atomicAdd(&sm.gl[2][k], zk * vvk + zj * vlk);
-      }
-      Gx[s] = gx;
-      Gy[s] = gy;
-      Gz[s] = gz;
-      atomicAdd(&sm.rho[ia[s]], rho_t);
-    }
-    __syncthreads();
+      double S[CT][5];
 #pragma unroll
-    for (int s = 0; s < PPL; ++s) {
-      const int p = tid + s * BLOCK;
-      if (ia[s] < 0) continue;
-      Gx[s] += sm.gl[0][p];
-      Gy[s] += sm.gl[1][p];
-      Gz[s] += sm.gl[2][p];
+      for (int c = 0; c < CT; ++c)
+#pragma unroll
+        for (int q = 0; q < 5; ++q) S[c][q] = 0.0;
+      for (int k = kb; k < ke; ++k) {
+        const double4 P = sm.rec[k];
+        const double cs = xj * P.x + yj * P.y + zj * P.z;
+        const GLoc L = g_locate<POINT, Q>(Hg, cs);
+        const bool zero = k == p || (cut && !(cs < Hg.xcut));
+#pragma unroll
+        for (int c = 0; c < CT; ++c) {
+          double gv, gp;
+          g_value<POINT, Q>(HDc(c)[args.pot.fn_off[4] + args.types[ch.a0 + ia[s]]], GTc(c) + goff, L, gv, gp);
+          const double fk = zero ? 0.0 : (c == 0 ? P.w : sm.fx[c > 0 ? c - 1 : 0][k]);
+          const double b = gp * fk;
+          S[c][0] += gv * fk;
+          S[c][1] += b * cs;
+          S[c][2] += b * P.x;
+          S[c][3] += b * P.y;
+          S[c][4] += b * P.z;
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < CT; ++c) {
+        sm.rp[c][p] += 0.5 * fv[s][c] * S[c][0];
+        const double fr = fv[s][c] * invr[s];
+        const double radial = dfv[s][c] * S[c][0] - fr * S[c][1];
+        Gx[s][c] = xj * radial + fr * S[c][2];
+        Gy[s][c] = yj * radial + fr * S[c][3];
+        Gz[s][c] = zj * radial + fr * S[c][4];
+      }
     }
   }
+  __syncthreads();
+  // per-atom densities from the staged pair contributions (two candidates at
+  // a time share one eight-lane pass)
+#pragma unroll
+  for (int c = 0; c < CT; c += 2) {
+    const int c2 = c + 1 < CT ? c + 1 : c;
+    atom_sums3<BLOCK>(
+        sm.poff, ch.na,
+        [&](int q, double& x, double& y, double& z) {
+          x = sm.rp[c][q];
+          y = sm.rp[c2][q];
+          z = 0.0;
+        },
+        [&](int a, double x, double y, double) {
+          sm.rho[c][a] = x;
+          sm.rho[c2][a] = y;
+        });
+  }
+  __syncthreads();
 
   // ---- phase 3: embedding
-  double c_acc = 0.0;
+  double c_acc[CT];
   const double wcell = args.weight[ch.cell];
+#pragma unroll
+  for (int c = 0; c < CT; ++c) c_acc[c] = 0.0;
   for (int a = tid; a < ch.na; a += BLOCK) {
-    double rho = sm.rho[a];
-    if (args.rho_out) args.rho_out[(size_t)cand * args.natoms + ch.a0 + a] = rho;
-    const SplHead& HU = HD[args.pot.fn_off[2] + args.types[ch.a0 + a]];
-    const double lo = HU.x0, hi = HU.xn;
-    if (rho < lo) {
-      c_acc += wcell * kDummyWeight * 10. * (lo - rho) * (lo - rho);
-      if (!args.pot.embed_extrap) rho = lo;
-    } else if (rho > hi) {
-      c_acc += wcell * kDummyWeight * 10. * (rho - hi) * (rho - hi);
-      if (!args.pot.embed_extrap) rho = hi;
+    const int su = args.pot.fn_off[2] + args.types[ch.a0 + a];
+#pragma unroll
+    for (int c = 0; c < CT; ++c) {
+      double rho = sm.rho[c][a];
+      if (args.rho_out && valid[c]) args.rho_out[(size_t)cand[c] * args.natoms + ch.a0 + a] = rho;
+      const SplHead& HU = HDc(c)[su];
+      const double lo = HU.x0, hi = HU.xn;
+      if (rho < lo) {
+        c_acc[c] += wcell * kDummyWeight * 10. * (lo - rho) * (lo - rho);
+        if (!args.pot.embed_extrap) rho = lo;
+      } else if (rho > hi) {
+        c_acc[c] += wcell * kDummyWeight * 10. * (rho - hi) * (rho - hi);
+        if (!args.pot.embed_extrap) rho = hi;
+      }
+      double dU, d2U;
+      e_acc[c] += eval_point<Q>(HU, IVc(c), rho, dU, d2U);
+      sm.dF[c][a] = dU;
+      if (args.dF_out && valid[c]) args.dF_out[(size_t)cand[c] * args.natoms + ch.a0 + a] = dU;
     }
-    double dU, d2U;
-    e_acc += eval_point<Q>(HU, IV, rho, dU, d2U);
-    sm.dF[a] = dU;
-    if (args.dF_out) args.dF_out[(size_t)cand * args.natoms + ch.a0 + a] = dU;
   }
   __syncthreads();
 
-  // ---- phase 4: pair forces, three-body forces, virial
-  double sxx = 0.0, syy = 0.0, szz = 0.0, sxy = 0.0, syz = 0.0, szx = 0.0;
-#pragma unroll
-  for (int s = 0; s < PPL; ++s) {
-    if (ia[s] < 0) continue;
-    const double dFi = sm.dF[ia[s]];
-    const double sc = aphi[s] + brho[s] * dFi;
-    const double tx = ux[s] * sc + dFi * Gx[s];
-    const double ty = uy[s] * sc + dFi * Gy[s];
-    const double tz = uz[s] * sc + dFi * Gz[s];
-    if (args.G_out) {
-      double* g = args.G_out + ((size_t)cand * args.npairs + ch.p0 + tid + s * BLOCK) * 3;
-      g[0] = Gx[s];
-      g[1] = Gy[s];
-      g[2] = Gz[s];
-    }
-    double* tv = args.tvec + ((size_t)cand * args.npairs + ch.p0 + tid + s * BLOCK) * 3;
-    tv[0] = tx;
-    tv[1] = ty;
-    tv[2] = tz;
-    atomicAdd(&sm.fi[ia[s]][0], tx);
-    atomicAdd(&sm.fi[ia[s]][1], ty);
-    atomicAdd(&sm.fi[ia[s]][2], tz);
-    const double rx = tx * r[s], ry = ty * r[s], rz = tz * r[s];
-    sxx -= ux[s] * rx;
-    syy -= uy[s] * ry;
-    szz -= uz[s] * rz;
-    sxy -= ux[s] * ry;
-    syz -= uy[s] * rz;
-    szx -= uz[s] * rx;
-  }
-  __syncthreads();
-  for (int a = tid; a < ch.na; a += BLOCK) {
-    double* fa = args.fown + ((size_t)cand * args.natoms + ch.a0 + a) * 3;
-    fa[0] = sm.fi[a][0];
-    fa[1] = sm.fi[a][1];
-    fa[2] = sm.fi[a][2];
-  }
-  // chunk partials of energy, virial and penalty: one wave-level reduction
-  // per value, then the per-wave sums in fixed order
-  const double vals[8] = {e_acc, sxx, syy, szz, sxy, syz, szx, c_acc};
+  // ---- phase 4: pair forces, three-body forces, virial, chunk partials
   const int lane = tid & 63, wid = tid >> 6;
 #pragma unroll
-  for (int q = 0; q < 8; ++q) {
-    const double v = wave_sum(vals[q]);
-    if (lane == 0) sm.red8[q][wid] = v;
+  for (int c = 0; c < CT; ++c) {
+    double vals[8] = {e_acc[c], 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, c_acc[c]};
+#pragma unroll
+    for (int s = 0; s < PPL; ++s) {
+      if (ia[s] < 0) continue;
+      const int p = tid + s * BLOCK;
+      const double dFi = sm.dF[c][ia[s]];
+      const double sc = aphi[s][c] + brho[s][c] * dFi;
+      const double tx = ux[s] * sc + dFi * Gx[s][c];
+      const double ty = uy[s] * sc + dFi * Gy[s][c];
+      const double tz = uz[s] * sc + dFi * Gz[s][c];
+      const size_t pg = (size_t)cand[c] * args.npairs + ch.p0 + p;
+      if (valid[c]) {
+        if (args.G_out) {
+          double* g = args.G_out + pg * 3;
+          g[0] = Gx[s][c];
+          g[1] = Gy[s][c];
+          g[2] = Gz[s][c];
+        }
+        double* tv = args.tvec + pg * 3;
+        tv[0] = tx;
+        tv[1] = ty;
+        tv[2] = tz;
+      }
+      sm.rec[p] = make_double4(tx, ty, tz, 0.0);
+      const double rx = tx * r[s], ry = ty * r[s], rz = tz * r[s];
+      vals[1] -= ux[s] * rx;
+      vals[2] -= uy[s] * ry;
+      vals[3] -= uz[s] * rz;
+      vals[4] -= ux[s] * ry;
+      vals[5] -= uy[s] * rz;
+      vals[6] -= uz[s] * rx;
+    }
+    // chunk partials of energy, virial and penalty
+    const double w8 = wave_sums8(vals);
+    if ((lane & 7) == 0) sm.red8[c][((lane >> 5) & 1) * 4 + ((lane >> 4) & 1) * 2 + ((lane >> 3) & 1)][wid] = w8;
+    __syncthreads();
+    // own-force sums: F_a gains T_p over its own pairs
+    atom_sums3<BLOCK>(
+        sm.poff, ch.na,
+        [&](int q, double& x, double& y, double& z) {
+          const double4 T = sm.rec[q];
+          x = T.x;
+          y = T.y;
+          z = T.z;
+        },
+        [&](int a, double x, double y, double z) {
+          if (!valid[c]) return;
+          double* fa = args.fown + ((size_t)cand[c] * args.natoms + ch.a0 + a) * 3;
+          fa[0] = x;
+          fa[1] = y;
+          fa[2] = z;
+        });
+    __syncthreads();
   }
-  __syncthreads();
-  if (tid < 8) {
+  if (tid < 8 * CT) {
+    const int c = tid >> 3, q = tid & 7;
     double t = 0.0;
 #pragma unroll
-    for (int w = 0; w < BLOCK / 64; ++w) t += sm.red8[tid][w];
-    args.chunk_acc[((size_t)cand * args.nchunks + blockIdx.x) * 8 + tid] = t;
+    for (int w = 0; w < BLOCK / 64; ++w) t += sm.red8[c][q][w];
+    if (blockIdx.y * CT + c < args.ncand)
+      args.chunk_acc[((size_t)(blockIdx.y * CT + c) * args.nchunks + blockIdx.x) * 8 + q] = t;
   }
 }
 
@@ -641,16 +820,16 @@ void launch_zero(hipStream_t st, double* p, size_t n) {
   hipLaunchKernelGGL(k_zero, dim3((unsigned)blocks), dim3(256), 0, st, p, n);
 }
 
-template <int BLOCK, int PPL, int MAXA>
+template <int BLOCK, int PPL, int MAXA, int CT>
 static void launch_atoms_t(hipStream_t st, const AtomArgs& a, int nchunks, int ncand) {
   const int style = a.pot.style;
   const bool meam = style == kMeamSpline || style == kMeam;
   const bool point = style == kEam || style == kMeam;
   const bool q = a.pot.quintic != 0;
-  const size_t dyn = (size_t)a.table_bytes + (size_t)a.pot.ngco * 48;
-  dim3 grid(nchunks, ncand), block(BLOCK);
+  const size_t dyn = atoms_dyn_lds(a.pot, CT);
+  dim3 grid(nchunks, (ncand + CT - 1) / CT), block(BLOCK);
 #define MZ_LAUNCH(M, P, Q_)                                                                          \
-  hipLaunchKernelGGL((k_atoms<BLOCK, PPL, MAXA, M, P, Q_>), grid, block, dyn, st, a)
+  hipLaunchKernelGGL((k_atoms<BLOCK, PPL, MAXA, CT, M, P, Q_>), grid, block, dyn, st, a)
   if (meam) {
     if (point) { if (q) MZ_LAUNCH(true, true, true); else MZ_LAUNCH(true, true, false); }
     else { if (q) MZ_LAUNCH(true, false, true); else MZ_LAUNCH(true, false, false); }
@@ -661,15 +840,34 @@ static void launch_atoms_t(hipStream_t st, const AtomArgs& a, int nchunks, int n
 #undef MZ_LAUNCH
 }
 
-int launch_atoms(hipStream_t st, const AtomArgs& a, int nchunks, int ncand, int block, int ppl) {
-  if (block == 256 && ppl == 2) launch_atoms_t<256, 2, 64>(st, a, nchunks, ncand);
-  else if (block == 256 && ppl == 1) launch_atoms_t<256, 1, 64>(st, a, nchunks, ncand);
-  else if (block == 128 && ppl == 2) launch_atoms_t<128, 2, 64>(st, a, nchunks, ncand);
-  else if (block == 128 && ppl == 4) launch_atoms_t<128, 4, 64>(st, a, nchunks, ncand);
-  else if (block == 512 && ppl == 1) launch_atoms_t<512, 1, 64>(st, a, nchunks, ncand);
-  else if (block == 64 && ppl == 4) launch_atoms_t<64, 4, 64>(st, a, nchunks, ncand);
-  else return -1;
-  return 0;
+size_t atoms_dyn_lds(const PotDesc& pot, int ct) {
+  const bool meam = pot.style == kMeamSpline || pot.style == kMeam;
+  return (size_t)ct * ((size_t)pot.table_bytes + (meam ? (size_t)pot.ngco * 8 * g_record_doubles(pot.quintic != 0) : 0));
+}
+
+size_t atoms_static_lds(int block, int ppl, int ct) {
+  const size_t P = (size_t)block * ppl;
+  return P * 32 + (size_t)(ct > 1 ? ct - 1 : 1) * P * 8 + (size_t)ct * P * 8 + (size_t)ct * 64 * 8 * 2 + 65 * 4 +
+         (size_t)ct * 8 * (block / 64) * 8;
+}
+
+int launch_atoms(hipStream_t st, const AtomArgs& a, int nchunks, int ncand, int block, int ppl, int ct) {
+#define MZ_SHAPE(B, P_)                                                       \
+  if (block == B && ppl == P_) {                                             \
+    if (ct == 1) launch_atoms_t<B, P_, 64, 1>(st, a, nchunks, ncand);        \
+    else if (ct == 2) launch_atoms_t<B, P_, 64, 2>(st, a, nchunks, ncand);   \
+    else if (ct == 4) launch_atoms_t<B, P_, 64, 4>(st, a, nchunks, ncand);   \
+    else return -1;                                                          \
+    return 0;                                                                \
+  }
+  MZ_SHAPE(256, 2)
+  MZ_SHAPE(256, 1)
+  MZ_SHAPE(128, 2)
+  MZ_SHAPE(128, 4)
+  MZ_SHAPE(512, 1)
+  MZ_SHAPE(64, 4)
+#undef MZ_SHAPE
+  return -1;
 }
 
 int resid_blocks(int natoms, int ncells) {

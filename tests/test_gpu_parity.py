@@ -90,3 +90,26 @@ def test_gpu_graph_replays_match_eager():
         Fb = obj.batch(K)                        # batched graph of 3 vectors
         Fe = np.array([obj.compute(k)["F"] for k in K])
         assert np.max(np.abs(Fb - Fe) / np.abs(Fe)) < 1e-12
+
+
+@pytest.mark.parametrize("cand_tile,ncand", [(2, 5), (4, 7), (4, 4), (2, 1)])
+def test_gpu_candidate_tiles_agree(cand_tile, ncand):
+    """Workgroups scoring several candidates of a batch (tails included) agree
+    with plain evaluations and with the oracle; so do their gradients."""
+    _say("candidate tile", cand_tile, ncand)
+    w = gen.workload(3, scale=16 / 1000)
+    gen.add_targets(w, pyoracle.evaluator)
+    obj = Objective(w.pot, w.db)
+    K = w.candidate_knots(ncand)
+    Fp = np.array([obj.objective(k) for k in K])
+    Gp = np.array([obj.objective_grad(k)[1] for k in K])
+    t = obj.get_tile()
+    obj.set_tile(t.max_atoms, t.max_pairs, t.block, cand_tile)
+    Fb, Gb = obj.batch(K, grad=True)
+    assert np.max(np.abs(Fb - Fp) / np.abs(Fp)) < 1e-12
+    assert np.max(np.abs(Fb - obj.batch(K)) / np.abs(Fp)) < 1e-12
+    for q in range(ncand):
+        assert rel_err(Gb[q], Gp[q]) < 1e-10, q
+    o = pyoracle.Oracle(w.pot, w.db)
+    Fo = np.array([o.objective(k) for k in K])
+    assert np.max(np.abs(Fb - Fo) / np.abs(Fo)) < TOL
