@@ -107,4 +107,59 @@ __device__ __forceinline__ bool eval_radial(const SplHead& H, const SplInterval*
   return eval_knot<Q>(H, IV + H.coef_off, r, v, d);
 }
 
+// Sums three per-pair values (vals[q] for q in the atom's pair range) for
+// every atom of the chunk, eight lanes per atom, fixed order; out(a, x, y, z)
+// receives the sums.  Every lane of the workgroup must call it.
+template <int BLOCK, class Get, class Out>
+__device__ __forceinline__ void atom_sums3(const int* poff, int na, Get get, Out out) {
+  const int l = threadIdx.x & 7;
+  for (int base = 0; base < na; base += BLOCK / 8) {
+    const int a = base + (int)(threadIdx.x >> 3);
+    double x = 0.0, y = 0.0, z = 0.0;
+    if (a < na) {
+      for (int q = poff[a] + l; q < poff[a + 1]; q += 8) {
+        double vx, vy, vz;
+        get(q, vx, vy, vz);
+        x += vx;
+        y += vy;
+        z += vz;
+      }
+    }
+#pragma unroll
+    for (int o = 4; o > 0; o >>= 1) {
+      x += __shfl_xor(x, o, 8);
+      y += __shfl_xor(y, o, 8);
+      z += __shfl_xor(z, o, 8);
+    }
+    if (a < na && l == 0) out(a, x, y, z);
+  }
+}
+
+// Wave totals of eight per-lane values with ten shuffles: three halving
+// exchanges leave each lane with one value index summed over eight lanes, three
+// more steps finish the sum.  Lane l with (l & 7) == 0 returns the total of
+// value index ((l >> 5) & 1) * 4 + ((l >> 4) & 1) * 2 + ((l >> 3) & 1).
+__device__ __forceinline__ double wave_sums8(const double (&v)[8]) {
+  const int lane = threadIdx.x & 63;
+  double a[4], b[2];
+  const bool h5 = lane & 32, h4 = lane & 16, h3 = lane & 8;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const double send = h5 ? v[i] : v[i + 4];
+    const double keep = h5 ? v[i + 4] : v[i];
+    a[i] = keep + __shfl_xor(send, 32, 64);
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const double send = h4 ? a[i] : a[i + 2];
+    const double keep = h4 ? a[i + 2] : a[i];
+    b[i] = keep + __shfl_xor(send, 16, 64);
+  }
+  double c = (h3 ? b[1] : b[0]) + __shfl_xor(h3 ? b[0] : b[1], 8, 64);
+  c += __shfl_xor(c, 4, 64);
+  c += __shfl_xor(c, 2, 64);
+  c += __shfl_xor(c, 1, 64);
+  return c;
+}
+
 }  // namespace mz
