@@ -133,7 +133,7 @@ typedef struct {
   int max_atoms;        /* atoms per chunk (work unit) */
   int max_pairs;        /* pairs per chunk */
   int block;            /* threads per workgroup */
-  int cand_tile;        /* candidates per workgroup in batched mode */
+  int cand_tile;        /* candidates per workgroup in batched mode: 1, 2 or 4 */
 } mz_tile;
 int mz_set_tile(mz_ctx* ctx, const mz_tile* tile);
 int mz_get_tile(const mz_ctx* ctx, mz_tile* tile);

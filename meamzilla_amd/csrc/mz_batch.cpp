@@ -30,24 +30,27 @@ int mz_autotune(mz_ctx* c, int ncand, int reps, mz_tile* best) {
   mz_get_tile(c, &cur);
   static const int shapes[][2] = {{256, 512}, {256, 256}, {128, 256}, {128, 512}, {512, 512}, {64, 256}};
   static const int atoms[] = {16, 32, 64};
+  static const int ctiles[] = {1, 2, 4};
   std::vector<double> kv((size_t)ncand * info.nknot, 0.0);
-  // any finite knot vector times the same work; use zeros except boundary slots
+  // the work per launch does not depend on the knot values: any finite vector times it
   std::vector<double> Fv(ncand);
   double best_t = 1e300;
   mz_tile bt = cur;
   for (auto& sh : shapes)
-    for (int na : atoms) {
-      mz_tile t{na, sh[1], sh[0], cur.cand_tile};
-      if (mz_set_tile(c, &t) != MZ_OK) continue;
-      if (mz_objective_batch(c, ncand, kv.data(), Fv.data(), nullptr) != MZ_OK) continue;
-      auto t0 = std::chrono::steady_clock::now();
-      for (int r = 0; r < reps; ++r) mz_objective_batch(c, ncand, kv.data(), Fv.data(), nullptr);
-      double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() / reps;
-      if (dt < best_t) {
-        best_t = dt;
-        bt = t;
+    for (int na : atoms)
+      for (int ct : ctiles) {
+        if (ct > 1 && ncand < ct) continue;
+        mz_tile t{na, sh[1], sh[0], ct};
+        if (mz_set_tile(c, &t) != MZ_OK) continue;
+        if (mz_objective_batch(c, ncand, kv.data(), Fv.data(), nullptr) != MZ_OK) continue;
+        auto t0 = std::chrono::steady_clock::now();
+        for (int r = 0; r < reps; ++r) mz_objective_batch(c, ncand, kv.data(), Fv.data(), nullptr);
+        double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() / reps;
+        if (dt < best_t) {
+          best_t = dt;
+          bt = t;
+        }
       }
-    }
   mz_set_tile(c, &bt);
   if (best) *best = bt;
   return MZ_OK;

@@ -96,7 +96,7 @@ struct mz_ctx {
   std::vector<mz::Chunk> chunks;
   std::vector<int> in_off, in_list;           // CSR: pairs whose neighbour is each atom
   std::vector<int> cell_chunk_off, cell_chunks; // CSR: chunks of every cell
-  mz_tile tile{64, 256, 256, 1};
+  mz_tile tile{64, 512, 256, 2};
   // device: static
   hipStream_t stream = nullptr;
   int device = 0;

@@ -113,3 +113,20 @@ def test_gpu_candidate_tiles_agree(cand_tile, ncand):
     o = pyoracle.Oracle(w.pot, w.db)
     Fo = np.array([o.objective(k) for k in K])
     assert np.max(np.abs(Fb - Fo) / np.abs(Fo)) < TOL
+
+
+def test_gpu_autotune_keeps_results():
+    """The autotuner picks a supported tile and the objective of every
+    candidate is unchanged under it."""
+    _say("autotune")
+    w = gen.workload(1, scale=8 / 256)
+    gen.add_targets(w, pyoracle.evaluator)
+    obj = Objective(w.pot, w.db)
+    K = w.candidate_knots(4)
+    F0 = obj.batch(K)
+    t = obj.autotune(ncand=4, reps=1)
+    assert t.cand_tile in (1, 2, 4) and t.block in (64, 128, 256, 512)
+    g = obj.get_tile()
+    assert (g.max_atoms, g.max_pairs, g.block, g.cand_tile) == (t.max_atoms, t.max_pairs, t.block, t.cand_tile)
+    F1 = obj.batch(K)
+    assert np.max(np.abs(F1 - F0) / np.abs(F0)) < 1e-12
