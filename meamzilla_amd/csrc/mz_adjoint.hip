@@ -808,6 +808,7 @@ int mz_grad_impl(mz_ctx* c, int ncand, const double* knots, double* F, double* G
   const size_t per_cand = (size_t)c->npairs * 6 * sizeof(double) + (size_t)c->natoms * 120 + (size_t)c->ncells * 64 +
                           c->chunks.size() * 64 + (size_t)mz::kSlots * X.ntab * 8;
   int sub = (int)std::max<size_t>(1, std::min<size_t>((size_t)ncand, (size_t)(48ull << 30) / std::max<size_t>(per_cand, 1)));
+  mz_timing acc{};  // kernel times summed over the sub-batches of a timed call
   for (int c0 = 0; c0 < ncand; c0 += sub) {
     const int nb = std::min(sub, ncand - c0);
     rc = mz_ensure_buffers(c, nb);
@@ -830,7 +831,12 @@ int mz_grad_impl(mz_ctx* c, int ncand, const double* knots, double* F, double* G
       float ms = 0;
       (void)hipEventElapsedTime(&ms, c->ev[5], c->ev[6]);
       mz_record_timing(c);
-      c->last.adjoint_ms = ms;
+      acc.atom_ms += c->last.atom_ms;
+      acc.resid_ms += c->last.resid_ms;
+      acc.refresh_ms += c->last.refresh_ms;
+      acc.total_ms += c->last.total_ms;
+      acc.adjoint_ms += ms;
+      c->last = acc;
     } else {
       hipGraphExec_t ex = nullptr;
       for (auto& g : X.graphs)

@@ -93,31 +93,26 @@ def _reps_for(m: int) -> Tuple[int, int, int]:
 
 
 def _min_dist_ok(frac: np.ndarray, box: np.ndarray, dmin: float) -> np.ndarray:
-    """Boolean per atom: True if the atom has no periodic neighbour closer than dmin."""
+    """Boolean per atom: True if the atom has no periodic neighbour closer than dmin.
+
+    A k-d tree over the atoms and their periodic images (enough image layers
+    to cover dmin in every lattice direction) counts the points within dmin
+    of each atom; the atom itself is the only one allowed."""
+    from scipy.spatial import cKDTree
     n = len(frac)
     heights = np.abs(np.linalg.det(box)) / np.linalg.norm(np.cross(box[[1, 2, 0]], box[[2, 0, 1]]), axis=1)
-    ok = np.ones(n, dtype=bool)
-    if heights.min() > 2.2 * dmin:
-        # minimum image suffices
-        for s in range(0, n, 512):
-            d = frac[s:s + 512, None, :] - frac[None, :, :]
-            d -= np.rint(d)
-            cart = d @ box
-            r2 = np.einsum("ijk,ijk->ij", cart, cart)
-            idx = np.arange(s, min(s + 512, n))
-            r2[np.arange(len(idx)), idx] = np.inf
-            ok[idx] &= r2.min(axis=1) >= dmin * dmin
-        return ok
-    rng = [int(math.ceil(dmin / h)) + 1 for h in heights]
-    shifts = np.stack(np.meshgrid(*[np.arange(-r, r + 1) for r in rng], indexing="ij"), -1).reshape(-1, 3)
-    for i in range(n):
-        d = frac[None, :, :] - frac[i][None, None, :] + shifts[:, None, :]
-        cart = d @ box
-        r2 = np.einsum("ijk,ijk->ij", cart, cart)
-        zero = np.all(shifts == 0, axis=1)
-        r2[zero, i] = np.inf
-        ok[i] = r2.min() >= dmin * dmin
-    return ok
+    reach = [int(math.ceil(dmin / h)) for h in heights]
+    shifts = np.stack(np.meshgrid(*[np.arange(-r, r + 1) for r in reach], indexing="ij"), -1).reshape(-1, 3)
+    wrapped = frac - np.floor(frac)
+    img = (wrapped[None, :, :] + shifts[:, None, :]).reshape(-1, 3)
+    # only images within dmin of the cell can be that close to an atom in it
+    margin = dmin / heights
+    keep = np.all((img > -margin) & (img < 1.0 + margin), axis=1)
+    tree = cKDTree(img[keep] @ box)
+    # distance < dmin is a violation; query_ball_point counts distance <= r
+    r = np.nextafter(dmin, 0.0)
+    counts = tree.query_ball_point(wrapped @ box, r=r, return_length=True)
+    return counts == 1
 
 
 def _displace(rng, frac, box, sigma, dmin=MIN_SEP, tries=200):
