@@ -688,7 +688,8 @@ Potential* lbfgs(const Potential& start, const OptOptions& opt) {
   const int M = std::max(1, opt.lbfgs_mem);
   std::vector<std::vector<double>> S, Y;
   std::vector<double> rho;
-  if (out) *out << "  iter        F              |g|            step" << std::endl;
+  if (out) *out << "  iter        F              |g|            step    seconds" << std::endl;
+  const auto t0 = std::chrono::steady_clock::now();
   for (int it = 0; it < opt.max_steps; ++it) {
     // two-loop recursion
     std::vector<double> qv = g, alpha(S.size());
@@ -776,7 +777,8 @@ Potential* lbfgs(const Potential& start, const OptOptions& opt) {
     for (int i = 0; i < n; ++i) gn += g[i] * g[i];
     if (out)
       *out << std::setw(6) << it + 1 << "  " << std::scientific << std::setprecision(8) << F << "  "
-           << std::sqrt(gn) << "  " << step << std::endl;
+           << std::sqrt(gn) << "  " << step << "  " << std::fixed << std::setprecision(4)
+           << std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() << std::endl;
     if (Fold - F <= opt.d_eps * std::max(1.0, std::abs(F)) && opt.d_eps > 0) break;
   }
   return pot;
@@ -975,10 +977,20 @@ int run_param_file(const std::string& param_path, std::ostream& out) {
   std::mt19937 rng(opt.seed);
   for (const auto& st : prm.list("optstyle")) {
     rng.seed(opt.seed);
+    const auto calls0 = Potential::get_ncalls();
+    const auto t0 = std::chrono::steady_clock::now();
+    auto summary = [&]() {
+      double best = pots[0]->get_error();
+      for (auto* p : pots) best = std::min(best, p->get_error());
+      out << "optimizer " << st << ": " << Potential::get_ncalls() - calls0 << " objective calls, " << std::fixed
+          << std::setprecision(3) << std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count()
+          << " s, best F = " << std::scientific << std::setprecision(10) << best << std::endl;
+    };
     if (st == "genalg") {
       auto res = genalg(pots, opt, rng);
       for (auto* p : pots) delete p;
       pots = res;
+      summary();
       continue;
     }
     for (auto*& p : pots) {
@@ -990,6 +1002,7 @@ int run_param_file(const std::string& param_path, std::ostream& out) {
       delete p;
       p = np;
     }
+    summary();
   }
   PotFile endf = pf;
   endf.pots.clear();

@@ -74,3 +74,22 @@ def test_driver_powell_decreases_objective(tmp_path):
 def test_driver_genalg_runs_batched(tmp_path):
     ini, fin, out, d = _run_fit(tmp_path, "eam_spline_c0", "genalg", ["max_steps 3", "pop_size 8", "seed 5"])
     assert "generation 3" in out and fin[0] <= ini[0] * 10
+
+
+@pytest.mark.gpu
+def test_fitrun_lbfgs_and_powell(tmp_path):
+    """End-to-end fit run: files in the reference's formats, the C++ driver,
+    L-BFGS on the adjoint gradient and Powell on the same objective."""
+    import io
+    import json
+    from contextlib import redirect_stdout
+    from meamzilla_amd import fitrun
+    buf = io.StringIO()
+    with redirect_stdout(buf):
+        assert fitrun.main(["--workload", "1", "--scale", str(6 / 256), "--lbfgs-steps", "8", "--powell-steps", "1",
+                            "--out", str(tmp_path)]) == 0
+    r = json.loads(buf.getvalue().strip().splitlines()[-1])
+    for style in ("lbfgs", "powell"):
+        assert r[style]["final_F"] < r[style]["initial_F"], (style, r[style])
+        assert r[style]["calls"] > 0 and r[style]["seconds"] >= 0
+    assert len(r["lbfgs"]["trajectory"]) >= 1
