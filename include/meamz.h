@@ -126,6 +126,13 @@ int mz_objective_grad(mz_ctx* ctx, const double* knots, double* F, double* dF);
  * annealing src/opt_simann.cpp:120-135 and the line searches. */
 int mz_objective_batch(mz_ctx* ctx, int ncand, const double* knots, double* F, double* G);
 
+/* Objective and residual vector of ncand knot vectors (rows of
+ * knots[ncand*nknot]) in one batched launch sequence: F[ncand],
+ * errvec[ncand*nerr] in the layout of mz_objective.
+ * No reference counterpart: replaces the ncoeff+1 sequential residual
+ * evaluations of Powell's Jacobian, gamma_init src/opt_powell.cpp:348-389. */
+int mz_residuals_batch(mz_ctx* ctx, int ncand, const double* knots, double* F, double* errvec);
+
 /* Work-decomposition tile: atoms per chunk limit, pairs per chunk limit and
  * candidates per block.  mz_autotune times candidate tiles on the current
  * database and keeps the fastest; mz_get_tile reports the tile in use. */

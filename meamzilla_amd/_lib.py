@@ -79,6 +79,7 @@ def lib() -> C.CDLL:
     L.mz_densities.argtypes = [vp, dp, dp]
     L.mz_objective_grad.argtypes = [vp, dp, dp, dp]
     L.mz_objective_batch.argtypes = [vp, C.c_int, dp, dp, dp]
+    L.mz_residuals_batch.argtypes = [vp, C.c_int, dp, dp, dp]
     L.mz_set_tile.argtypes = [vp, C.POINTER(Tile)]
     L.mz_get_tile.argtypes = [vp, C.POINTER(Tile)]
     L.mz_autotune.argtypes = [vp, C.c_int, C.c_int, C.POINTER(Tile)]
@@ -187,6 +188,15 @@ class Objective:
         G = np.zeros((n, self.info.nknot)) if grad else None
         _check(lib().mz_objective_batch(self._ctx, n, _dp(kv), _dp(F), _dp(G)))
         return (F, G) if grad else F
+
+    def residuals_batch(self, knots: np.ndarray):
+        """(F[n], residual vectors [n, nerr]) of n knot vectors in one call."""
+        kv = self._kv(np.atleast_2d(knots))
+        n = kv.shape[0]
+        F = np.zeros(n)
+        E = np.zeros((n, self.info.nerr))
+        _check(lib().mz_residuals_batch(self._ctx, n, _dp(kv), _dp(F), _dp(E)))
+        return F, E
 
     def set_tile(self, max_atoms: int, max_pairs: int, block: int, cand_tile: int = 1):
         t = Tile(max_atoms, max_pairs, block, cand_tile)

@@ -13,12 +13,17 @@ int mz_run_forward(mz_ctx* c, int ncand, double* d_errvec, double* d_ce, double*
                    double* d_G, double* d_lam, double* d_cadj);
 int mz_stage_knots(mz_ctx* c, int ncand, const double* knots);
 int mz_batch_impl(mz_ctx* c, int ncand, const double* knots, double* F, double* G);
+int mz_residuals_batch_impl(mz_ctx* c, int ncand, const double* knots, double* F, double* errvec);
 int mz_ensure_buffers(mz_ctx* c, int ncand);
 
 extern "C" {
 
 int mz_objective_batch(mz_ctx* c, int ncand, const double* knots, double* F, double* G) {
   return mz_batch_impl(c, ncand, knots, F, G);
+}
+
+int mz_residuals_batch(mz_ctx* c, int ncand, const double* knots, double* F, double* errvec) {
+  return mz_residuals_batch_impl(c, ncand, knots, F, errvec);
 }
 
 int mz_autotune(mz_ctx* c, int ncand, int reps, mz_tile* best) {

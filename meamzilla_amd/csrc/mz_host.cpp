@@ -867,4 +867,31 @@ int mz_batch_impl(mz_ctx* c, int ncand, const double* knots, double* F, double* 
   return MZ_OK;
 }
 
+// Residual vectors of ncand knot vectors: the forward launch sequence with the
+// residual output enabled for every candidate, in sub-batches that keep the
+// per-candidate buffers and residual vectors within a fixed share of HBM.
+int mz_residuals_batch_impl(mz_ctx* c, int ncand, const double* knots, double* F, double* errvec) {
+  if (!c || !knots || !F || !errvec || ncand < 1) return fail(MZ_ERR_ARG, "null argument or ncand < 1");
+  const size_t per = (size_t)c->npairs * 24 + (size_t)c->natoms * 72 + c->chunks.size() * 64 +
+                     (size_t)c->nknot * 8 + (size_t)c->table_bytes + (size_t)c->nerr * 8 + 64;
+  const int sub = (int)std::max<size_t>(1, std::min<size_t>((size_t)ncand, (48ull << 30) / per));
+  int rc = ensure_eval_buffers(c, sub);
+  if (rc) return rc;
+  if (c->d_errvec.alloc((size_t)sub * c->nerr)) return fail(MZ_ERR_NOMEM, "residual buffer allocation failed");
+  hipStream_t st = c->stream;
+  for (int c0 = 0; c0 < ncand; c0 += sub) {
+    const int nb = std::min(sub, ncand - c0);
+    rc = mz_stage_knots(c, nb, knots + (size_t)c0 * c->nknot);
+    if (rc) return rc;
+    rc = mz_run_forward(c, nb, c->d_errvec.p, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
+    if (rc) return rc;
+    MZ_HIP(hipMemcpyAsync(F + c0, c->d_F.p, sizeof(double) * nb, hipMemcpyDeviceToHost, st));
+    MZ_HIP(hipMemcpyAsync(errvec + (size_t)c0 * c->nerr, c->d_errvec.p, sizeof(double) * nb * c->nerr,
+                          hipMemcpyDeviceToHost, st));
+    MZ_HIP(hipStreamSynchronize(st));
+    mz_record_timing(c);
+  }
+  return MZ_OK;
+}
+
 int mz_fail(int code, const std::string& msg) { return fail(code, msg); }

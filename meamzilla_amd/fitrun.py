@@ -68,6 +68,9 @@ def _parse(text: str) -> dict:
         # Powell outer step: inner steps, calls, F
         elif len(f) == 3 and f[0].isdigit() and f[1].isdigit():
             out["trajectory"].append({"inner": int(f[0]), "calls": int(f[1]), "F": float(f[2])})
+        # genetic algorithm: generation g  best F = x
+        elif len(f) == 6 and f[0] == "generation":
+            out["trajectory"].append({"generation": int(f[1]), "F": float(f[5])})
     return out
 
 
@@ -77,6 +80,12 @@ def main(argv=None) -> int:
     ap.add_argument("--scale", type=float, default=1.0)
     ap.add_argument("--lbfgs-steps", type=int, default=200)
     ap.add_argument("--powell-steps", type=int, default=2)
+    ap.add_argument("--styles", default="lbfgs,powell", help="optimizers to run, comma separated "
+                    "(lbfgs, powell, simann, genalg)")
+    ap.add_argument("--ga-pop", type=int, default=256, help="genetic algorithm population")
+    ap.add_argument("--ga-gens", type=int, default=5, help="genetic algorithm generations")
+    ap.add_argument("--ga-init-scale", type=float, default=0.1,
+                    help="relative size of the random knot changes of the initial population")
     ap.add_argument("--out", required=True)
     ap.add_argument("--free-gauge", action="store_true",
                     help="keep every knot free (Powell then meets the singular gauge directions)")
@@ -95,9 +104,16 @@ def main(argv=None) -> int:
           file=sys.stderr, flush=True)
     result = {"workload": w.name, "cells": w.db.ncells, "atoms": w.db.total_atoms, "ncoeff": w.pot.ncoeff(),
               "gauge": "free" if args.free_gauge else "one knot of every rho and f spline fixed"}
-    for style, steps in (("lbfgs", args.lbfgs_steps), ("powell", args.powell_steps)):
+    steps_of = {"lbfgs": args.lbfgs_steps, "powell": args.powell_steps, "simann": args.lbfgs_steps,
+                "genalg": args.ga_gens}
+    for style in args.styles.split(","):
+        steps = steps_of[style]
+        extra = [f"optstyle {style}", f"max_steps {steps}"]
+        if style == "genalg":
+            extra += [f"pop_size {args.ga_pop}", f"init_scale {args.ga_init_scale}", "cross_rate 0.5",
+                      "mut_rate 0.05", "seed 1"]
         param = os.path.join(args.out, f"param_{style}")
-        formats.write_param(param, "config", "pot", extra=[f"optstyle {style}", f"max_steps {steps}"])
+        formats.write_param(param, "config", "pot", extra=extra)
         ts = time.time()
         run = subprocess.run([DRIVER, "-p", param], capture_output=True, text=True)
         with open(os.path.join(args.out, f"{style}.log"), "w") as fh:

@@ -2,6 +2,8 @@
 #include "mzhost.h"
 
 #include <algorithm>
+#include <atomic>
+#include <thread>
 #include <chrono>
 #include <cmath>
 #include <fstream>
@@ -555,43 +557,74 @@ Potential* powell(const Potential& start, const OptOptions& opt) {
       p(ncoeff), q(ncoeff), delta(ncoeff);
   auto G = [&](int j, int i) -> double& { return gamma[(size_t)i * esize + j]; };
   auto D = [&](int i, int j) -> double& { return d[(size_t)i * ncoeff + j]; };
+  // Forward-difference Jacobian of the residual vector, one column per
+  // coefficient (src/opt_powell.cpp:348-389); the ncoeff perturbed residual
+  // vectors come from batched calls of at most kJacBatch knot vectors.
+  constexpr int kJacBatch = 64;
   auto gamma_init = [&](const Potential* base) -> int {
-    Potential tmp(*base);
     for (int i = 0; i < ncoeff; ++i)
       for (int j = 0; j < ncoeff; ++j) D(i, j) = (i == j) ? 1. : 0.;
-    for (int i = 0; i < ncoeff; ++i) {
-      ErrorVec tv;
-      const double store = tmp.coeff(i);
-      tmp.coeff(i) += P_EPS;
-      tmp.compute(&tv);
-      double sum = 0.0;
-      for (int j = 0; j < esize; ++j) {
-        const double t = (tv[j] - ev[j]) / P_EPS;
-        G(j, i) = t;
-        sum += t * t;
+    const std::vector<double> kv0 = base->knot_vector();
+    const std::vector<int>& slots = base->coeff_slots();
+    const size_t nk = kv0.size();
+    mz_ctx* ctx = base->evaluator()->ctx();
+    std::vector<double> kb, Fb, Eb;
+    for (int i0 = 0; i0 < ncoeff; i0 += kJacBatch) {
+      const int nb = std::min(kJacBatch, ncoeff - i0);
+      kb.assign((size_t)nb * nk, 0.0);
+      for (int b = 0; b < nb; ++b) {
+        std::copy(kv0.begin(), kv0.end(), kb.begin() + (size_t)b * nk);
+        kb[(size_t)b * nk + slots[i0 + b]] += P_EPS;
       }
-      const double mag = std::sqrt(sum);
-      tmp.coeff(i) = store;
-      if (mag < P_NOTHING) return i + 1;
-      for (int j = 0; j < esize; ++j) G(j, i) /= mag;
-      D(i, i) /= mag;
+      Fb.resize(nb);
+      Eb.resize((size_t)nb * esize);
+      if (mz_residuals_batch(ctx, nb, kb.data(), Fb.data(), Eb.data()) != MZ_OK)
+        throw Error(std::string("mz_residuals_batch: ") + mz_last_error());
+      Potential::add_ncalls(nb);
+      for (int b = 0; b < nb; ++b) {
+        const int i = i0 + b;
+        const double* tv = Eb.data() + (size_t)b * esize;
+        double sum = 0.0;
+        for (int j = 0; j < esize; ++j) {
+          const double t = (tv[j] - ev[j]) / P_EPS;
+          G(j, i) = t;
+          sum += t * t;
+        }
+        const double mag = std::sqrt(sum);
+        if (mag < P_NOTHING) return i + 1;
+        for (int j = 0; j < esize; ++j) G(j, i) /= mag;
+        D(i, i) /= mag;
+      }
     }
     return 0;
   };
+  // Normal equations of the Gauss-Newton step: p = -Gamma^T e and
+  // les = Gamma^T Gamma (all of it, or one row and column).  The full product
+  // is O(ncoeff^2 esize) and is split over host threads by column.
   auto lineqsys = [&](int only_col) {
-    for (int i = 0; i < ncoeff; ++i) {
+    auto column = [&](int i) {
       p[i] = 0.;
       for (int j = 0; j < esize; ++j) p[i] -= G(j, i) * ev[j];
-    }
-    for (int i = 0; i < ncoeff; ++i) {
-      if (only_col >= 0 && i != only_col) continue;
+      if (only_col >= 0 && i != only_col) return;
       for (int j = (only_col >= 0 ? 0 : i); j < ncoeff; ++j) {
         double s = 0.0;
         for (int k = 0; k < esize; ++k) s += G(k, i) * G(k, j);
         les[(size_t)i * ncoeff + j] = s;
         les[(size_t)j * ncoeff + i] = s;
       }
+    };
+    const int nth = only_col >= 0 ? 1 : (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    if (nth == 1) {
+      for (int i = 0; i < ncoeff; ++i) column(i);
+      return;
     }
+    std::atomic<int> next{0};
+    std::vector<std::thread> th;
+    for (int t = 0; t < nth; ++t)
+      th.emplace_back([&]() {
+        for (int i = next++; i < ncoeff; i = next++) column(i);
+      });
+    for (auto& x : th) x.join();
   };
   if (out) *out << std::setw(10) << " " << std::setw(3) << "n" << std::setw(8) << "ncalls" << "   F" << std::endl;
   double F_older = F;
@@ -886,6 +919,7 @@ std::vector<Potential*> genalg(const std::vector<Potential*>& pop_in, const OptO
       std::copy(kv.begin(), kv.end(), K.begin() + i * nknot);
     }
     check(mz_objective_batch(evaluator->ctx(), (int)ps.size(), K.data(), F.data(), nullptr));
+    Potential::add_ncalls(ps.size());
     for (size_t i = 0; i < ps.size(); ++i) ps[i]->set_error(F[i]);
   };
   score(pop);

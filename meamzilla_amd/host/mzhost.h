@@ -108,6 +108,7 @@ class Potential {
   double get_error() const { return error_sum_; }
   void set_error(double F) { error_sum_ = F; }
   static unsigned long long get_ncalls() { return ncalls_; }
+  static void add_ncalls(unsigned long long n) { ncalls_ += n; }
   static void reset_ncalls() { ncalls_ = 0; }
 
   std::vector<double> knot_vector() const;

@@ -130,3 +130,19 @@ def test_gpu_autotune_keeps_results():
     assert (g.max_atoms, g.max_pairs, g.block, g.cand_tile) == (t.max_atoms, t.max_pairs, t.block, t.cand_tile)
     F1 = obj.batch(K)
     assert np.max(np.abs(F1 - F0) / np.abs(F0)) < 1e-12
+
+
+def test_gpu_residuals_batch_matches_plain():
+    """Batched residual vectors (Powell's Jacobian columns) equal the plain
+    evaluation's residual vector for every candidate."""
+    _say("residuals batch")
+    w = gen.workload(3, scale=12 / 1000)
+    gen.add_targets(w, pyoracle.evaluator)
+    obj = Objective(w.pot, w.db)
+    K = w.candidate_knots(5)
+    F, E = obj.residuals_batch(K)
+    for q in range(len(K)):
+        r = obj.compute(K[q])
+        assert abs(F[q] - r["F"]) <= 1e-12 * abs(r["F"])
+        assert rel_err(E[q], r["errvec"]) < 1e-12, q
+        assert abs(np.dot(E[q], E[q]) - F[q]) <= 1e-10 * F[q]
