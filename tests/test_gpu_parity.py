@@ -146,3 +146,51 @@ def test_gpu_residuals_batch_matches_plain():
         assert abs(F[q] - r["F"]) <= 1e-12 * abs(r["F"])
         assert rel_err(E[q], r["errvec"]) < 1e-12, q
         assert abs(np.dot(E[q], E[q]) - F[q]) <= 1e-10 * F[q]
+
+
+def _cells_db(cells):
+    """Database from (positions (n,3) cartesian, box (3,3), types) triples."""
+    from meamzilla_amd.formats import Database
+    natoms = np.array([len(c[0]) for c in cells], np.int32)
+    box = np.array([c[1] for c in cells], float)
+    pos = np.concatenate([np.asarray(c[0], float) for c in cells])
+    types = np.concatenate([np.asarray(c[2], np.int32) for c in cells])
+    nc, na = len(cells), len(pos)
+    return Database(natoms, box, types, pos, np.zeros((na, 3)), np.zeros(nc), np.zeros((nc, 6)),
+                    np.ones(nc), np.ones(nc), np.ones(nc), 1.0, 1.0)
+
+
+@pytest.mark.parametrize("cfg", [0, 1, 2])
+def test_gpu_edge_geometries(cfg):
+    """Isolated atom (no neighbours), dimer, trimer, and one atom whose only
+    neighbours are its own periodic images, in one database; objective,
+    residuals and forces against the oracle, and the adjoint against central
+    differences."""
+    _say("edge geometries", cfg)
+    w = gen.workload(cfg, scale=0.01)
+    nt = w.pot.ntypes
+    L = 20.0
+    big = np.diag([L, L, L])
+    cells = [
+        (np.array([[5.0, 5.0, 5.0]]), big, [0]),
+        (np.array([[5.0, 5.0, 5.0], [7.9, 5.1, 5.0]]), big, [0, nt - 1]),
+        (np.array([[5.0, 5.0, 5.0], [7.7, 5.0, 5.0], [6.3, 7.3, 5.2]]), big, [0, nt - 1, 0]),
+        (np.array([[0.3, 0.2, 0.1]]), np.array([[2.9, 0.0, 0.0], [0.2, 3.0, 0.0], [0.1, -0.3, 3.1]]), [nt - 1]),
+    ]
+    db = _cells_db(cells)
+    gen.add_targets(gen.Workload("edge", w.pot, w.truth, db, w.cutoff, 1, 5), pyoracle.evaluator)
+    o = pyoracle.Oracle(w.pot, db)
+    obj = Objective(w.pot, db)
+    assert (obj.info.npairs, obj.info.ntriplets) == o.counts()
+    e = o.eval()
+    r = obj.compute()
+    assert abs(r["F"] - e["F"]) <= TOL * abs(e["F"])
+    for k in ("errvec", "cell_energy", "cell_stress", "forces"):
+        assert rel_err(r[k], e[k]) < TOL, k
+    kv = w.pot.knot_vector()
+    F, g = obj.objective_grad(kv)
+    h = 1e-6 * np.maximum(np.abs(kv), 1.0)
+    idx = np.arange(len(kv))
+    gcd = np.array([(obj.objective(kv + h[i] * (idx == i)) - obj.objective(kv - h[i] * (idx == i))) / (2 * h[i])
+                    for i in idx])
+    assert rel_err(g, gcd) < 1e-5
