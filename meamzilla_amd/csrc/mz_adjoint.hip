@@ -76,11 +76,10 @@ struct ChainArgs {
   double* grad;            // [cand][nknot]
 };
 
-struct Partner {
-  double4 a;  // unit vector, f
-  double4 b;  // a f', w / r (3)
-  double2 c;  // a / r, unused
-};
+// Partner data of every pair, structure of arrays (consecutive partners of
+// consecutive lanes read consecutive doubles): unit vector (3), f, a f',
+// w / r (3), a / r.
+constexpr int kPartnerFields = 9;
 
 // Interval-coefficient adjoint: weight al on the value, bt on d(value)/dt.
 template <bool Q>
@@ -156,7 +155,7 @@ __device__ __forceinline__ void acc_radial(double* tab, int ncoef, int s, const 
 template <bool MEAM, bool POINT, bool Q>
 __global__ void __launch_bounds__(kAdjBlock) k_adjoint(const AdjArgs A) {
   extern __shared__ __attribute__((aligned(64))) char dyn[];
-  __shared__ Partner pd[kAdjBlock];
+  __shared__ double pd[kPartnerFields][kAdjBlock];
   __shared__ double push[2][kAdjBlock];
   __shared__ double S[kAdjAtoms], psi[kAdjAtoms], Up[kAdjAtoms];
   __shared__ int poff[kAdjAtoms + 1];
@@ -246,11 +245,9 @@ __global__ void __launch_bounds__(kAdjBlock) k_adjoint(const AdjArgs A) {
     if (MEAM) {
       const double* G = A.G + ((size_t)cand * A.npairs + ch.p0 + p) * 3;
       sv += wx * G[0] + wy * G[1] + wz * G[2];
-      Partner P;
-      P.a = make_double4(ux, uy, uz, fj);
-      P.b = make_double4(aj * dfj, invr * wx, invr * wy, invr * wz);
-      P.c = make_double2(invr * aj, 0.0);
-      pd[p] = P;
+      const double pv[kPartnerFields] = {ux, uy, uz, fj, aj * dfj, invr * wx, invr * wy, invr * wz, invr * aj};
+#pragma unroll
+      for (int q = 0; q < kPartnerFields; ++q) pd[q][p] = pv[q];
       push[0][p] = push[1][p] = 0.0;
     }
     atomicAdd(&S[ia], sv);
@@ -308,8 +305,9 @@ __global__ void __launch_bounds__(kAdjBlock) k_adjoint(const AdjArgs A) {
     for (int m = 0; m < trips; ++m) {
       kl = (kl + 1 == n) ? 0 : kl + 1;
       const int k = kb + kl;
-      const double4 Pa = pd[k].a, Pb = pd[k].b;
-      const double Ck = pd[k].c.x;
+      const double4 Pa = make_double4(pd[0][k], pd[1][k], pd[2][k], pd[3][k]);
+      const double4 Pb = make_double4(pd[4][k], pd[5][k], pd[6][k], pd[7][k]);
+      const double Ck = pd[8][k];
       const double cs = ux * Pa.x + uy * Pa.y + uz * Pa.z;
       double gv, dgt, t;
       int fl;
@@ -679,7 +677,7 @@ int setup_adjoint(mz_ctx* c) {
   // the largest that fits the 64 KB workgroup limit.
   const int nc_g = c->pot.quintic ? 6 : 4;
   const int fixed = c->table_bytes + (X.ngco * 6 + X.ntab) * (int)sizeof(double);
-  const int stat = (int)(sizeof(mz::Partner) * mz::kAdjBlock + 2 * sizeof(double) * mz::kAdjBlock +
+  const int stat = (int)(mz::kPartnerFields * sizeof(double) * mz::kAdjBlock + 2 * sizeof(double) * mz::kAdjBlock +
                          3 * sizeof(double) * mz::kAdjAtoms + sizeof(int) * (mz::kAdjAtoms + 1));
   X.grep_log2 = -1;
   for (int lim : {40 * 1024, 64 * 1024}) {
