@@ -172,13 +172,13 @@ __global__ void __launch_bounds__(kAdjBlock) k_adjoint(const AdjArgs A) {
   }
   const SplHead* HD = reinterpret_cast<const SplHead*>(dyn);
   const SplInterval* IV = reinterpret_cast<const SplInterval*>(dyn + sizeof(SplHead) * A.pot.nspl);
-  double* gt = reinterpret_cast<double*>(dyn + A.table_bytes);   // g coefficients, 48-byte records
+  double* gt = reinterpret_cast<double*>(dyn + A.table_bytes);   // g coefficients, [coef][record]
   double* atab = gt + (MEAM ? A.ngco * 6 : 0);                    // coefficient + head adjoints
   double* gacc = atab + A.ntab;                                   // [g record][coef][replica]
   if (MEAM) {
     const SplInterval* src = reinterpret_cast<const SplInterval*>(A.tables + (size_t)cand * A.table_bytes +
                                                                   sizeof(SplHead) * A.pot.nspl);
-    for (int q = tid; q < A.ngco * 6; q += kAdjBlock) gt[q] = src[A.gco0 + q / 6].c[q % 6];
+    for (int q = tid; q < A.ngco * 6; q += kAdjBlock) gt[q] = src[A.gco0 + q % A.ngco].c[q / A.ngco];
   }
   const int nacc = A.ntab + (MEAM ? A.ngco * NC * R : 0);
   for (int q = tid; q < nacc; q += kAdjBlock) atab[q] = 0.0;
@@ -296,7 +296,8 @@ __global__ void __launch_bounds__(kAdjBlock) k_adjoint(const AdjArgs A) {
     const double gy0 = Hg.y0, gyn = Hg.yn, ggL = Hg.gL, ggR = Hg.gR, gg2L = Hg.g2L, gg2R = Hg.g2R;
     const bool gq = Hg.basis == 5;
     const int grec = Hg.coef_off - A.gco0;
-    const double* gtg = gt + (size_t)grec * 6;
+    const double* gtg = gt + grec;  // coefficient m of record r at gtg[m * ngco + r]
+    const int GN = A.ngco;
     double* gacl = gacc + (size_t)grec * NC * R + (tid & (R - 1));
     double* ghead = atab + ncoef * 6 + sg * 6;
     const double Aj = aj * dfj, Wjx = invr * wx, Wjy = invr * wy, Wjz = invr * wz, Cj = invr * aj;
@@ -326,10 +327,10 @@ __global__ void __launch_bounds__(kAdjBlock) k_adjoint(const AdjArgs A) {
         const double flo = fmin(fmax(floor(u), 0.0), gkmax);
         t = u - flo;
         fl = (int)flo;
-        const double2* C = reinterpret_cast<const double2*>(gtg + 6 * fl);
-        const double2 c01 = C[0], c23 = C[1];
+        const double* C = gtg + fl;
+        const double2 c01 = make_double2(C[0], C[GN]), c23 = make_double2(C[2 * GN], C[3 * GN]);
         if (Q) {
-          const double2 c45 = C[2];
+          const double2 c45 = make_double2(C[4 * GN], C[5 * GN]);
           gv = c01.x + t * (c01.y + t * (c23.x + t * (c23.y + t * (c45.x + t * c45.y))));
           dgt = c01.y + t * (2.0 * c23.x + t * (3.0 * c23.y + t * (4.0 * c45.x + t * 5.0 * c45.y)));
         } else {
