@@ -56,6 +56,9 @@ struct AdjArgs {
   int ntab;                // ncoef*6 + nspl*6
   int gco0, ngco;          // first g interval record, number of g interval records
   int grep_log2;           // log2 of the lane replicas of the g-coefficient adjoints
+  int nrad;                // interval records below the g records (phi, rho, U, f)
+  int rrep_log2;           // log2 of the lane replicas of their adjoints, -1: none
+  int rrep_stride;         // doubles per replica (nrad NC padded to 1 mod 32: replicas on distinct banks)
 };
 
 struct ChainArgs {
@@ -81,27 +84,36 @@ struct ChainArgs {
 // w / r (3), a / r.
 constexpr int kPartnerFields = 9;
 
+// Where interval-coefficient adjoints go: coefficient m of interval record rec
+// at base[rec * rs + m * cs] (the workgroup table: rs 6, cs 1; a lane replica
+// of the radial records: rs NC, cs 1, base at the replica's start).
+struct CoefSink {
+  double* base;
+  int rs, cs;
+};
+
 // Interval-coefficient adjoint: weight al on the value, bt on d(value)/dt.
 template <bool Q>
-__device__ __forceinline__ void acc_interval(double* tab, int rec, double t, double al, double bt) {
-  double* c = tab + rec * 6;
+__device__ __forceinline__ void acc_interval(const CoefSink& sk, int rec, double t, double al, double bt) {
+  double* c = sk.base + rec * sk.rs;
+  const int cs = sk.cs;
   const double t2 = t * t, t3 = t2 * t;
-  atomicAdd(c + 0, al);
-  atomicAdd(c + 1, al * t + bt);
-  atomicAdd(c + 2, al * t2 + 2.0 * bt * t);
-  atomicAdd(c + 3, al * t3 + 3.0 * bt * t2);
+  atomicAdd(c, al);
+  atomicAdd(c + cs, al * t + bt);
+  atomicAdd(c + 2 * cs, al * t2 + 2.0 * bt * t);
+  atomicAdd(c + 3 * cs, al * t3 + 3.0 * bt * t2);
   if (Q) {
     const double t4 = t3 * t;
-    atomicAdd(c + 4, al * t4 + 4.0 * bt * t3);
-    atomicAdd(c + 5, al * t4 * t + 5.0 * bt * t4);
+    atomicAdd(c + 4 * cs, al * t4 + 4.0 * bt * t3);
+    atomicAdd(c + 5 * cs, al * t4 * t + 5.0 * bt * t4);
   }
 }
 
 // Adjoint of a point evaluation (value weight al, d/dr weight be) including
 // the end extrapolation; head slots per spline: y0, yn, gL, gR, g2L, g2R.
 template <bool Q>
-__device__ __forceinline__ void acc_point(double* tab, int ncoef, int s, const SplHead& H, double r, double al,
-                                          double be) {
+__device__ __forceinline__ void acc_point(double* tab, const CoefSink& sk, int ncoef, int s, const SplHead& H,
+                                          double r, double al, double be) {
   double* hd = tab + ncoef * 6 + s * 6;
   if (r < H.x0) {
     const double e = r - H.x0;
@@ -122,7 +134,7 @@ __device__ __forceinline__ void acc_point(double* tab, int ncoef, int s, const S
   if (k >= H.n - 1) k = H.n - 2;
   if (k < 0) k = 0;
   const double t = __dmul_rn(__dadd_rn(delta, -__dmul_rn((double)k, H.step)), H.invstep);
-  acc_interval<Q>(tab, H.coef_off + k, t, al, be * H.invstep);
+  acc_interval<Q>(sk, H.coef_off + k, t, al, be * H.invstep);
 }
 
 // Knot-style location (same rule as eval_knot); false beyond x_n (1 + 1e-8).
@@ -139,17 +151,17 @@ __device__ __forceinline__ bool locate_knot(const SplHead& H, const SplInterval*
 
 // Adjoint of one radial function of a pair (value weight al, d/dr weight be).
 template <bool POINT, bool Q>
-__device__ __forceinline__ void acc_radial(double* tab, int ncoef, int s, const SplHead& H, const SplInterval* IV,
-                                           double r, double al, double be) {
+__device__ __forceinline__ void acc_radial(double* tab, const CoefSink& sk, int ncoef, int s, const SplHead& H,
+                                           const SplInterval* IV, double r, double al, double be) {
   if (POINT) {
     if (!(r <= H.xn)) return;
-    acc_point<Q>(tab, ncoef, s, H, r, al, be);
+    acc_point<Q>(tab, sk, ncoef, s, H, r, al, be);
     return;
   }
   int k;
   double t;
   if (!locate_knot(H, IV + H.coef_off, r, k, t)) return;
-  acc_interval<Q>(tab, H.coef_off + k, t, al, be * H.invstep);
+  acc_interval<Q>(sk, H.coef_off + k, t, al, be * H.invstep);
 }
 
 template <bool MEAM, bool POINT, bool Q>
@@ -157,7 +169,8 @@ __global__ void __launch_bounds__(kAdjBlock) k_adjoint(const AdjArgs A) {
   extern __shared__ __attribute__((aligned(64))) char dyn[];
   __shared__ double pd[kPartnerFields][kAdjBlock];
   __shared__ double push[2][kAdjBlock];
-  __shared__ double S[kAdjAtoms], psi[kAdjAtoms], Up[kAdjAtoms];
+  __shared__ double psi[kAdjAtoms], Up[kAdjAtoms];
+  static_assert(kAdjAtoms * 4 == kAdjBlock, "phase B takes four lanes per atom");
   __shared__ int poff[kAdjAtoms + 1];
   constexpr int NC = Q ? 6 : 4;  // coefficients per g interval that carry adjoints
   const int cand = blockIdx.y;
@@ -183,7 +196,6 @@ __global__ void __launch_bounds__(kAdjBlock) k_adjoint(const AdjArgs A) {
   const int nacc = A.ntab + (MEAM ? A.ngco * NC * R : 0);
   for (int q = tid; q < nacc; q += kAdjBlock) atab[q] = 0.0;
   for (int a = tid; a <= ch.na; a += kAdjBlock) poff[a] = A.atom_poff[ch.a0 + a] - ch.p0;
-  for (int a = tid; a < ch.na; a += kAdjBlock) S[a] = 0.0;
   __syncthreads();
 
   const int nt = A.pot.nt;
@@ -248,14 +260,26 @@ __global__ void __launch_bounds__(kAdjBlock) k_adjoint(const AdjArgs A) {
       const double pv[kPartnerFields] = {ux, uy, uz, fj, aj * dfj, invr * wx, invr * wy, invr * wz, invr * aj};
 #pragma unroll
       for (int q = 0; q < kPartnerFields; ++q) pd[q][p] = pv[q];
-      push[0][p] = push[1][p] = 0.0;
+      push[0][p] = 0.0;
     }
-    atomicAdd(&S[ia], sv);
+    push[1][p] = sv;  // staged for S_i; cleared again below
   }
   __syncthreads();
 
-  // ---- phase B: per-atom density sensitivity and the embedding adjoint
-  for (int a = tid; a < ch.na; a += kAdjBlock) {
+  // ---- phase B: S_i summed by four lanes per atom in a fixed order (each
+  // staged entry read and cleared by one lane), then the atom's density
+  // sensitivity and embedding adjoint.
+  const int ba = tid >> 2;
+  double Sa = 0.0;
+  if (ba < ch.na)
+    for (int q = poff[ba] + (tid & 3); q < poff[ba + 1]; q += 4) {
+      Sa += push[1][q];
+      push[1][q] = 0.0;
+    }
+  Sa += __shfl_xor(Sa, 1, 4);
+  Sa += __shfl_xor(Sa, 2, 4);
+  if (ba < ch.na && (tid & 3) == 0) {
+    const int a = ba;
     const double rho = A.rho[(size_t)cand * A.natoms + ch.a0 + a];
     const int su = A.pot.fn_off[2] + A.types[ch.a0 + a];
     const SplHead& HU = HD[su];
@@ -266,11 +290,11 @@ __global__ void __launch_bounds__(kAdjBlock) k_adjoint(const AdjArgs A) {
     double dU, d2U;
     (void)eval_point<Q>(HU, IV, rt, dU, d2U);
     double ps = 0.0;
-    if (!out || A.pot.embed_extrap) ps = lamE * dU + S[a] * d2U;
+    if (!out || A.pot.embed_extrap) ps = lamE * dU + Sa * d2U;
     if (out) ps += kappa * wcell * kDummyWeight * 10. * 2.0 * (rho - bound);
     psi[a] = ps;
     Up[a] = dU;
-    acc_point<Q>(atab, ncoef, su, HU, rt, lamE, S[a]);
+    acc_point<Q>(atab, CoefSink{atab, 6, 1}, ncoef, su, HU, rt, lamE, Sa);
   }
   __syncthreads();
 
@@ -379,32 +403,45 @@ __global__ void __launch_bounds__(kAdjBlock) k_adjoint(const AdjArgs A) {
     Phi = Rj;
     Phip = Qj;
   }
-  if (MEAM) __syncthreads();
+  // ---- phase D: pair functions.  Pairs of similar length hit the same
+  // interval, so their coefficient adjoints go to RR lane replicas of the
+  // radial records, laid out [replica][record][coef] in the partner-data
+  // area, which the sweep no longer reads.
+  __syncthreads();
+  const int RR = A.rrep_log2 >= 0 ? 1 << A.rrep_log2 : 0;
+  const int RS = A.rrep_stride;
+  double* rrep = &pd[0][0];
+  for (int q = tid; q < RS * RR; q += kAdjBlock) rrep[q] = 0.0;
   if (MEAM && has) {
     Phi += push[0][p];
     Phip = Up[ia] * aj * (Phip + push[1][p]);
   }
-
-  // ---- phase D: pair functions
+  __syncthreads();
   if (has) {
+    const CoefSink sk = RR ? CoefSink{rrep + (tid & (RR - 1)) * RS, NC, 1} : CoefSink{atab, 6, 1};
     const double ps = psi[ia], up = Up[ia];
     const int sphi = A.pot.fn_off[0] + pij, srho = A.pot.fn_off[1] + tj;
-    acc_radial<POINT, Q>(atab, ncoef, sphi, HD[sphi], IV, r, 0.5 * lamE, 0.5 * aj);
-    acc_radial<POINT, Q>(atab, ncoef, srho, HD[srho], IV, r, ps, aj * up);
+    acc_radial<POINT, Q>(atab, sk, ncoef, sphi, HD[sphi], IV, r, 0.5 * lamE, 0.5 * aj);
+    acc_radial<POINT, Q>(atab, sk, ncoef, srho, HD[srho], IV, r, ps, aj * up);
     if (MEAM) {
       const int sf = A.pot.fn_off[3] + pij;
-      acc_radial<POINT, Q>(atab, ncoef, sf, HD[sf], IV, r, Phi, Phip);
+      acc_radial<POINT, Q>(atab, sk, ncoef, sf, HD[sf], IV, r, Phi, Phip);
     }
   }
   __syncthreads();
+  for (int q = tid; q < A.nrad * NC * (RR ? 1 : 0); q += kAdjBlock) {
+    double sacc = 0.0;
+    for (int rep = 0; rep < RR; ++rep) sacc += rrep[rep * RS + q];
+    atab[(q / NC) * 6 + q % NC] += sacc;
+  }
   if (MEAM) {
     for (int q = tid; q < A.ngco * NC; q += kAdjBlock) {
       double sacc = 0.0;
       for (int rep = 0; rep < R; ++rep) sacc += gacc[(size_t)q * R + rep];
       atab[(A.gco0 + q / NC) * 6 + q % NC] += sacc;
     }
-    __syncthreads();
   }
+  __syncthreads();
   double* slot = A.slots + ((size_t)cand * kSlots + (blockIdx.x % kSlots)) * A.ntab;
   for (int q = tid; q < A.ntab; q += kAdjBlock) {
     const double v = atab[q];
@@ -679,7 +716,7 @@ int setup_adjoint(mz_ctx* c) {
   const int nc_g = c->pot.quintic ? 6 : 4;
   const int fixed = c->table_bytes + (X.ngco * 6 + X.ntab) * (int)sizeof(double);
   const int stat = (int)(mz::kPartnerFields * sizeof(double) * mz::kAdjBlock + 2 * sizeof(double) * mz::kAdjBlock +
-                         3 * sizeof(double) * mz::kAdjAtoms + sizeof(int) * (mz::kAdjAtoms + 1));
+                         2 * sizeof(double) * mz::kAdjAtoms + sizeof(int) * (mz::kAdjAtoms + 1));
   X.grep_log2 = -1;
   for (int lim : {40 * 1024, 64 * 1024}) {
     for (int lg = 5; lg >= 0 && X.grep_log2 < 0; --lg)
@@ -687,6 +724,12 @@ int setup_adjoint(mz_ctx* c) {
     if (X.grep_log2 >= 0) break;
   }
   if (X.grep_log2 < 0) return mz_fail(MZ_ERR_CAPACITY, "adjoint workgroup would exceed 64 KB of LDS");
+  // Lane replicas of the radial interval records reuse the partner-data area.
+  X.nrad = c->meam ? X.gco0 : c->ncoef;
+  X.rrep_stride = ((X.nrad * nc_g + 31) & ~31) + 1;
+  X.rrep_log2 = -1;
+  for (int lg = 4; lg >= 0 && X.rrep_log2 < 0; --lg)
+    if (X.rrep_stride * (1 << lg) <= mz::kPartnerFields * mz::kAdjBlock) X.rrep_log2 = lg;
   X.dyn_bytes = fixed + (c->meam ? X.ngco * nc_g * (1 << X.grep_log2) * 8 : 0);
   X.ready = true;
   return MZ_OK;
@@ -756,6 +799,9 @@ static int run_grad(mz_ctx* c, int ncand) {
   a.gco0 = X.gco0;
   a.ngco = X.ngco;
   a.grep_log2 = X.grep_log2;
+  a.nrad = X.nrad;
+  a.rrep_log2 = X.rrep_log2;
+  a.rrep_stride = X.rrep_stride;
   const bool q = c->pot.quintic != 0;
   if (c->meam) {
     if (c->point) { if (q) launch_adjoint_t<true, true, true>(st, a, X.nchunks, ncand, X.dyn_bytes);

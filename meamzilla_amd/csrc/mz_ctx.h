@@ -61,7 +61,8 @@ struct Grid {
 struct AdjointExtra {
   bool ready = false;
   int cap = 0;
-  int nchunks = 0, ntab = 0, gco0 = 0, ngco = 0, dyn_bytes = 0, grep_log2 = 0;
+  int nchunks = 0, ntab = 0, gco0 = 0, ngco = 0, dyn_bytes = 0, grep_log2 = 0, nrad = 0, rrep_log2 = -1,
+      rrep_stride = 0;
   DevBuf<Chunk> d_chunks;                 // chunks of <= 256 pairs
   DevBuf<double> d_J;                     // refresh Jacobians
   DevBuf<int> d_joff;
