@@ -39,6 +39,7 @@ struct AdjArgs {
   PotDesc pot;
   const Chunk* chunks;
   const int* atom_poff;
+  const int* owner;        // [npairs] atom that owns the pair
   const uint2* pairs;
   const double* pos;
   const double* box;
@@ -164,7 +165,9 @@ __device__ __forceinline__ void acc_radial(double* tab, const CoefSink& sk, int 
   acc_interval<Q>(sk, H.coef_off + k, t, al, be * H.invstep);
 }
 
-template <bool MEAM, bool POINT, bool Q>
+// GR: lane replicas of the g-coefficient adjoints (32 keeps the 32 lanes that
+// one LDS cycle serves on distinct replicas; fewer when LDS is short).
+template <bool MEAM, bool POINT, bool Q, int GR>
 __global__ void __launch_bounds__(kAdjBlock) k_adjoint(const AdjArgs A) {
   extern __shared__ __attribute__((aligned(64))) char dyn[];
   __shared__ double pd[kPartnerFields][kAdjBlock];
@@ -177,28 +180,34 @@ __global__ void __launch_bounds__(kAdjBlock) k_adjoint(const AdjArgs A) {
   const int tid = threadIdx.x;
   const Chunk ch = A.chunks[blockIdx.x];
   const int ncoef = A.pot.ncoef;
-  const int R = 1 << A.grep_log2;
-  {
-    const int4* src = reinterpret_cast<const int4*>(A.tables + (size_t)cand * A.table_bytes);
-    int4* dst = reinterpret_cast<int4*>(dyn);
-    for (int q = tid; q < A.table_bytes / 16; q += kAdjBlock) dst[q] = src[q];
-  }
-  const SplHead* HD = reinterpret_cast<const SplHead*>(dyn);
-  const SplInterval* IV = reinterpret_cast<const SplInterval*>(dyn + sizeof(SplHead) * A.pot.nspl);
-  double* gt = reinterpret_cast<double*>(dyn + A.table_bytes);   // g coefficients, [coef][record]
-  double* atab = gt + (MEAM ? A.ngco * 6 : 0);                    // coefficient + head adjoints
-  double* gacc = atab + A.ntab;                                   // [g record][coef][replica]
-  if (MEAM) {
-    const SplInterval* src = reinterpret_cast<const SplInterval*>(A.tables + (size_t)cand * A.table_bytes +
-                                                                  sizeof(SplHead) * A.pot.nspl);
-    for (int q = tid; q < A.ngco * 6; q += kAdjBlock) gt[q] = src[A.gco0 + q % A.ngco].c[q / A.ngco];
-  }
-  const int nacc = A.ntab + (MEAM ? A.ngco * NC * R : 0);
-  for (int q = tid; q < nacc; q += kAdjBlock) atab[q] = 0.0;
-  for (int a = tid; a <= ch.na; a += kAdjBlock) poff[a] = A.atom_poff[ch.a0 + a] - ch.p0;
-  __syncthreads();
-
+  constexpr int R = GR;
   const int nt = A.pot.nt;
+  const int p = tid;
+  const bool has = p < ch.np;
+  const int ba = tid >> 2;  // atom of this lane in phase B
+
+  // Every global load of phases A and B is issued here, beside the table
+  // staging, so that one barrier covers their latency: pair record, owner and
+  // leg sums, then positions, types and force duals of both ends.
+  uint2 pr = make_uint2(0u, 0u);
+  int ig = ch.a0;
+  double G0 = 0, G1 = 0, G2 = 0;
+  if (has) {
+    pr = A.pairs[ch.p0 + p];
+    ig = A.owner[ch.p0 + p];
+    if (MEAM) {
+      const double* G = A.G + ((size_t)cand * A.npairs + ch.p0 + p) * 3;
+      G0 = G[0];
+      G1 = G[1];
+      G2 = G[2];
+    }
+  }
+  double rho_b = 0.0;
+  int type_b = 0;
+  if (ba < ch.na) {
+    rho_b = A.rho[(size_t)cand * A.natoms + ch.a0 + ba];
+    type_b = A.types[ch.a0 + ba];
+  }
   const double* box = A.box + 9 * ch.cell;
   const double* lam = A.lam + (size_t)cand * A.natoms * 3;
   const double* cadj = A.cell_adj + ((size_t)cand * A.ncells + ch.cell) * 8;
@@ -206,41 +215,55 @@ __global__ void __launch_bounds__(kAdjBlock) k_adjoint(const AdjArgs A) {
   const double mxx = cadj[1], myy = cadj[2], mzz = cadj[3], mxy = cadj[4], myz = cadj[5], mzx = cadj[6];
   const double kappa = cadj[7];
   const double wcell = A.weight[ch.cell];
-
-  // ---- phase A: pair duals, S_i, partner data
-  const int p = tid;
-  const bool has = p < ch.np;
-  int ia = 0, ti = 0, pij = 0, tj = 0;
-  double r = 0, invr = 0, ux = 0, uy = 0, uz = 0, fj = 0, dfj = 0, drho = 0, aj = 0, wx = 0, wy = 0, wz = 0;
+  const int j = (int)pr.x;
+  int ti = 0, tj = 0;
+  double d[3] = {0.0, 0.0, 0.0}, lw[3] = {0.0, 0.0, 0.0};
   if (has) {
-    int lo = 0, hi = ch.na - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (poff[mid] <= p) lo = mid; else hi = mid - 1;
-    }
-    ia = lo;
-    const int ig = ch.a0 + lo;
-    const uint2 pr = A.pairs[ch.p0 + p];
-    const int j = (int)pr.x;
     const int ix = (int)(int8_t)(pr.y & 0xff), iy = (int)(int8_t)((pr.y >> 8) & 0xff),
               iz = (int)(int8_t)((pr.y >> 16) & 0xff);
     const double* pj = A.pos + 3 * (size_t)j;
     const double* pi = A.pos + 3 * (size_t)ig;
-    double d[3];
 #pragma unroll
     for (int m = 0; m < 3; ++m) {
       double t = __dadd_rn(pj[m], __dmul_rn((double)ix, box[m]));
       t = __dadd_rn(t, __dmul_rn((double)iy, box[3 + m]));
       t = __dadd_rn(t, __dmul_rn((double)iz, box[6 + m]));
       d[m] = __dadd_rn(t, -pi[m]);
+      lw[m] = lam[3 * (size_t)ig + m] - lam[3 * (size_t)j + m];
     }
+    ti = A.types[ig];
+    tj = A.types[j];
+  }
+  {
+    const int4* src = reinterpret_cast<const int4*>(A.tables + (size_t)cand * A.table_bytes);
+    int4* dst = reinterpret_cast<int4*>(dyn);
+    for (int q = tid; q < A.table_bytes / 16; q += kAdjBlock) dst[q] = src[q];
+  }
+  const SplHead* HD = reinterpret_cast<const SplHead*>(dyn);
+  const SplInterval* IV = reinterpret_cast<const SplInterval*>(dyn + sizeof(SplHead) * A.pot.nspl);
+  double* gt = reinterpret_cast<double*>(dyn + A.table_bytes);   // g coefficients, [record][NC]
+  double* atab = gt + (MEAM ? A.ngco * NC : 0);                   // coefficient + head adjoints
+  double* gacc = atab + A.ntab;                                   // [g record][coef][replica]
+  if (MEAM) {
+    const SplInterval* src = reinterpret_cast<const SplInterval*>(A.tables + (size_t)cand * A.table_bytes +
+                                                                  sizeof(SplHead) * A.pot.nspl);
+    for (int q = tid; q < A.ngco * NC; q += kAdjBlock) gt[q] = src[A.gco0 + q / NC].c[q % NC];
+  }
+  const int nacc = A.ntab + (MEAM ? A.ngco * NC * R : 0);
+  for (int q = tid; q < nacc; q += kAdjBlock) atab[q] = 0.0;
+  for (int a = tid; a <= ch.na; a += kAdjBlock) poff[a] = A.atom_poff[ch.a0 + a] - ch.p0;
+  __syncthreads();
+
+  // ---- phase A: pair duals, S_i, partner data
+  const int ia = ig - ch.a0;
+  int pij = 0;
+  double r = 0, invr = 0, ux = 0, uy = 0, uz = 0, fj = 0, dfj = 0, drho = 0, aj = 0, wx = 0, wy = 0, wz = 0;
+  if (has) {
     r = __dsqrt_rn(__dadd_rn(__dadd_rn(__dmul_rn(d[0], d[0]), __dmul_rn(d[1], d[1])), __dmul_rn(d[2], d[2])));
     invr = 1.0 / r;
     ux = d[0] / r;
     uy = d[1] / r;
     uz = d[2] / r;
-    ti = A.types[ig];
-    tj = A.types[j];
     pij = pair_ij(ti, tj, nt);
     double v, dv;
     if (eval_radial<POINT, Q>(HD[A.pot.fn_off[1] + tj], IV, r, v, dv)) drho = dv;
@@ -249,14 +272,13 @@ __global__ void __launch_bounds__(kAdjBlock) k_adjoint(const AdjArgs A) {
       dfj = dv;
     }
     const double vx = mxx * ux + mzx * uz, vy = myy * uy + mxy * ux, vz = mzz * uz + myz * uy;
-    wx = lam[3 * (size_t)ig] - lam[3 * (size_t)j] - r * vx;
-    wy = lam[3 * (size_t)ig + 1] - lam[3 * (size_t)j + 1] - r * vy;
-    wz = lam[3 * (size_t)ig + 2] - lam[3 * (size_t)j + 2] - r * vz;
+    wx = lw[0] - r * vx;
+    wy = lw[1] - r * vy;
+    wz = lw[2] - r * vz;
     aj = wx * ux + wy * uy + wz * uz;
     double sv = aj * drho;
     if (MEAM) {
-      const double* G = A.G + ((size_t)cand * A.npairs + ch.p0 + p) * 3;
-      sv += wx * G[0] + wy * G[1] + wz * G[2];
+      sv += wx * G0 + wy * G1 + wz * G2;
       const double pv[kPartnerFields] = {ux, uy, uz, fj, aj * dfj, invr * wx, invr * wy, invr * wz, invr * aj};
 #pragma unroll
       for (int q = 0; q < kPartnerFields; ++q) pd[q][p] = pv[q];
@@ -269,7 +291,6 @@ __global__ void __launch_bounds__(kAdjBlock) k_adjoint(const AdjArgs A) {
   // ---- phase B: S_i summed by four lanes per atom in a fixed order (each
   // staged entry read and cleared by one lane), then the atom's density
   // sensitivity and embedding adjoint.
-  const int ba = tid >> 2;
   double Sa = 0.0;
   if (ba < ch.na)
     for (int q = poff[ba] + (tid & 3); q < poff[ba + 1]; q += 4) {
@@ -280,8 +301,8 @@ __global__ void __launch_bounds__(kAdjBlock) k_adjoint(const AdjArgs A) {
   Sa += __shfl_xor(Sa, 2, 4);
   if (ba < ch.na && (tid & 3) == 0) {
     const int a = ba;
-    const double rho = A.rho[(size_t)cand * A.natoms + ch.a0 + a];
-    const int su = A.pot.fn_off[2] + A.types[ch.a0 + a];
+    const double rho = rho_b;
+    const int su = A.pot.fn_off[2] + type_b;
     const SplHead& HU = HD[su];
     const bool below = rho < HU.x0, above = rho > HU.xn;
     const bool out = below || above;
@@ -320,9 +341,8 @@ __global__ void __launch_bounds__(kAdjBlock) k_adjoint(const AdjArgs A) {
     const double gy0 = Hg.y0, gyn = Hg.yn, ggL = Hg.gL, ggR = Hg.gR, gg2L = Hg.g2L, gg2R = Hg.g2R;
     const bool gq = Hg.basis == 5;
     const int grec = Hg.coef_off - A.gco0;
-    const double* gtg = gt + grec;  // coefficient m of record r at gtg[m * ngco + r]
-    const int GN = A.ngco;
-    double* gacl = gacc + (size_t)grec * NC * R + (tid & (R - 1));
+    const double2* gtg = reinterpret_cast<const double2*>(gt + grec * NC);
+    double* gacl = gacc + grec * NC * R + (tid & (R - 1));
     double* ghead = atab + ncoef * 6 + sg * 6;
     const double Aj = aj * dfj, Wjx = invr * wx, Wjy = invr * wy, Wjz = invr * wz, Cj = invr * aj;
     double Rj = 0.0, Qj = 0.0;
@@ -351,10 +371,10 @@ __global__ void __launch_bounds__(kAdjBlock) k_adjoint(const AdjArgs A) {
         const double flo = fmin(fmax(floor(u), 0.0), gkmax);
         t = u - flo;
         fl = (int)flo;
-        const double* C = gtg + fl;
-        const double2 c01 = make_double2(C[0], C[GN]), c23 = make_double2(C[2 * GN], C[3 * GN]);
+        const double2* C = gtg + fl * (NC / 2);
+        const double2 c01 = C[0], c23 = C[1];
         if (Q) {
-          const double2 c45 = make_double2(C[4 * GN], C[5 * GN]);
+          const double2 c45 = C[2];
           gv = c01.x + t * (c01.y + t * (c23.x + t * (c23.y + t * (c45.x + t * c45.y))));
           dgt = c01.y + t * (2.0 * c23.x + t * (3.0 * c23.y + t * (4.0 * c45.x + t * 5.0 * c45.y)));
         } else {
@@ -388,7 +408,7 @@ __global__ void __launch_bounds__(kAdjBlock) k_adjoint(const AdjArgs A) {
       } else {
         const double bt = be * ginv;
         const double t2 = t * t, t3 = t2 * t;
-        double* cp = gacl + (size_t)fl * NC * R;
+        double* cp = gacl + fl * (NC * R);
         atomicAdd(cp, al);
         atomicAdd(cp + R, al * t + bt);
         atomicAdd(cp + 2 * R, al * t2 + 2.0 * bt * t);
@@ -705,22 +725,27 @@ int setup_adjoint(mz_ctx* c) {
   std::vector<mz::Chunk> sorted(ch.size());
   for (size_t q = 0; q < ch.size(); ++q) sorted[q] = ch[order[q]];
   if (X.d_chunks.upload(sorted.data(), sorted.size())) return mz_fail(MZ_ERR_NOMEM, "adjoint chunk upload failed");
+  std::vector<int> owner((size_t)c->npairs);
+  for (int a = 0; a < c->natoms; ++a)
+    for (int q = c->atom_poff[a]; q < c->atom_poff[a + 1]; ++q) owner[q] = a;
+  if (X.d_owner.upload(owner.data(), owner.size())) return mz_fail(MZ_ERR_NOMEM, "adjoint owner upload failed");
   X.nchunks = (int)sorted.size();
   X.ntab = c->ncoef * 6 + c->nspl * 6;
   X.gco0 = c->meam ? c->coef_off[c->pot.fn_off[4]] : 0;
   X.ngco = c->meam ? c->ncoef - X.gco0 : 0;
   // LDS: spline table, compact g coefficients, coefficient adjoints and the
   // lane replicas of the g-coefficient adjoints.  The replica count is the
-  // largest power of two that keeps four workgroups per CU (40 KB each), else
-  // the largest that fits the 64 KB workgroup limit.
+  // largest of 32, 8, 1 (the counts the kernel is built for) that keeps four
+  // workgroups per CU (40 KB each), else the largest that fits the 64 KB
+  // workgroup limit.
   const int nc_g = c->pot.quintic ? 6 : 4;
-  const int fixed = c->table_bytes + (X.ngco * 6 + X.ntab) * (int)sizeof(double);
+  const int fixed = c->table_bytes + (X.ngco * nc_g + X.ntab) * (int)sizeof(double);
   const int stat = (int)(mz::kPartnerFields * sizeof(double) * mz::kAdjBlock + 2 * sizeof(double) * mz::kAdjBlock +
                          2 * sizeof(double) * mz::kAdjAtoms + sizeof(int) * (mz::kAdjAtoms + 1));
   X.grep_log2 = -1;
   for (int lim : {40 * 1024, 64 * 1024}) {
-    for (int lg = 5; lg >= 0 && X.grep_log2 < 0; --lg)
-      if (fixed + (c->meam ? X.ngco * nc_g * (1 << lg) * 8 : 0) + stat <= lim) X.grep_log2 = lg;
+    for (int lg : {5, 3, 0})  // the replica counts the kernel is built for
+      if (X.grep_log2 < 0 && fixed + (c->meam ? X.ngco * nc_g * (1 << lg) * 8 : 0) + stat <= lim) X.grep_log2 = lg;
     if (X.grep_log2 >= 0) break;
   }
   if (X.grep_log2 < 0) return mz_fail(MZ_ERR_CAPACITY, "adjoint workgroup would exceed 64 KB of LDS");
@@ -756,7 +781,11 @@ int ensure_adjoint_buffers(mz_ctx* c, int ncand) {
 
 template <bool M, bool P, bool Q>
 void launch_adjoint_t(hipStream_t st, const mz::AdjArgs& a, int nchunks, int ncand, int dyn) {
-  hipLaunchKernelGGL((mz::k_adjoint<M, P, Q>), dim3(nchunks, ncand), dim3(mz::kAdjBlock), dyn, st, a);
+  const dim3 grid(nchunks, ncand), block(mz::kAdjBlock);
+  if (!M) hipLaunchKernelGGL((mz::k_adjoint<M, P, Q, 1>), grid, block, dyn, st, a);
+  else if (a.grep_log2 == 5) hipLaunchKernelGGL((mz::k_adjoint<M, P, Q, 32>), grid, block, dyn, st, a);
+  else if (a.grep_log2 == 3) hipLaunchKernelGGL((mz::k_adjoint<M, P, Q, 8>), grid, block, dyn, st, a);
+  else hipLaunchKernelGGL((mz::k_adjoint<M, P, Q, 1>), grid, block, dyn, st, a);
 }
 
 }  // namespace
@@ -780,6 +809,7 @@ static int run_grad(mz_ctx* c, int ncand) {
   a.pot = c->pot;
   a.chunks = X.d_chunks.p;
   a.atom_poff = c->d_atom_poff.p;
+  a.owner = X.d_owner.p;
   a.pairs = c->d_pairs.p;
   a.pos = c->d_pos.p;
   a.box = c->d_box.p;
@@ -921,6 +951,7 @@ void mz_adjoint_drop(mz_ctx* c) {
   X.graphs.clear();
   X.d_rho.release(); X.d_dF.release(); X.d_G.release(); X.d_lam.release(); X.d_cadj.release();
   X.d_slots.release(); X.d_grad.release(); X.d_J.release(); X.d_joff.release(); X.d_chunks.release();
+  X.d_owner.release();
   if (X.h_grad) (void)hipHostFree(X.h_grad);
   X.h_grad = nullptr;
   X.cap = 0;

@@ -64,6 +64,7 @@ struct AdjointExtra {
   int nchunks = 0, ntab = 0, gco0 = 0, ngco = 0, dyn_bytes = 0, grep_log2 = 0, nrad = 0, rrep_log2 = -1,
       rrep_stride = 0;
   DevBuf<Chunk> d_chunks;                 // chunks of <= 256 pairs
+  DevBuf<int> d_owner;                    // owning atom of every pair
   DevBuf<double> d_J;                     // refresh Jacobians
   DevBuf<int> d_joff;
   DevBuf<double> d_rho, d_dF, d_G, d_lam, d_cadj, d_slots, d_grad;
