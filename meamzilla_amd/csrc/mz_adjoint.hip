@@ -234,14 +234,13 @@ __global__ void __launch_bounds__(kAdjBlock) k_adjoint(const AdjArgs A) {
     ti = A.types[ig];
     tj = A.types[j];
   }
-  {
-    const int4* src = reinterpret_cast<const int4*>(A.tables + (size_t)cand * A.table_bytes);
-    int4* dst = reinterpret_cast<int4*>(dyn);
-    for (int q = tid; q < A.table_bytes / 16; q += kAdjBlock) dst[q] = src[q];
-  }
-  const SplHead* HD = reinterpret_cast<const SplHead*>(dyn);
-  const SplInterval* IV = reinterpret_cast<const SplInterval*>(dyn + sizeof(SplHead) * A.pot.nspl);
-  double* gt = reinterpret_cast<double*>(dyn + A.table_bytes);   // g coefficients, [record][NC]
+  // The candidate's spline table stays in global memory (cache-resident; it
+  // is read per pair and per atom, not per triplet), so LDS goes to the
+  // accumulators; only the g coefficients, read per triplet, are staged.
+  const char* tbl = A.tables + (size_t)cand * A.table_bytes;
+  const SplHead* HD = reinterpret_cast<const SplHead*>(tbl);
+  const SplInterval* IV = reinterpret_cast<const SplInterval*>(tbl + sizeof(SplHead) * A.pot.nspl);
+  double* gt = reinterpret_cast<double*>(dyn);                    // g coefficients, [record][NC]
   double* atab = gt + (MEAM ? A.ngco * NC : 0);                   // coefficient + head adjoints
   double* gacc = atab + A.ntab;                                   // [g record][coef][replica]
   if (MEAM) {
@@ -733,21 +732,19 @@ int setup_adjoint(mz_ctx* c) {
   X.ntab = c->ncoef * 6 + c->nspl * 6;
   X.gco0 = c->meam ? c->coef_off[c->pot.fn_off[4]] : 0;
   X.ngco = c->meam ? c->ncoef - X.gco0 : 0;
-  // LDS: spline table, compact g coefficients, coefficient adjoints and the
-  // lane replicas of the g-coefficient adjoints.  The replica count is the
-  // largest of 32, 8, 1 (the counts the kernel is built for) that keeps four
-  // workgroups per CU (40 KB each), else the largest that fits the 64 KB
-  // workgroup limit.
+  // LDS: compact g coefficients, coefficient adjoints and the lane replicas
+  // of the g-coefficient adjoints.  The replica count is the largest of 32,
+  // 8, 1 (the counts the kernel is built for) that fits the 64 KB workgroup
+  // limit: replicas (fewer lanes adding into one address) were measured to
+  // matter more than workgroups per CU.
   const int nc_g = c->pot.quintic ? 6 : 4;
-  const int fixed = c->table_bytes + (X.ngco * nc_g + X.ntab) * (int)sizeof(double);
+  const int fixed = (X.ngco * nc_g + X.ntab) * (int)sizeof(double);
   const int stat = (int)(mz::kPartnerFields * sizeof(double) * mz::kAdjBlock + 2 * sizeof(double) * mz::kAdjBlock +
                          2 * sizeof(double) * mz::kAdjAtoms + sizeof(int) * (mz::kAdjAtoms + 1));
   X.grep_log2 = -1;
-  for (int lim : {40 * 1024, 64 * 1024}) {
-    for (int lg : {5, 3, 0})  // the replica counts the kernel is built for
-      if (X.grep_log2 < 0 && fixed + (c->meam ? X.ngco * nc_g * (1 << lg) * 8 : 0) + stat <= lim) X.grep_log2 = lg;
-    if (X.grep_log2 >= 0) break;
-  }
+  for (int lg : {5, 3, 0})  // the replica counts the kernel is built for
+    if (X.grep_log2 < 0 && fixed + (c->meam ? X.ngco * nc_g * (1 << lg) * 8 : 0) + stat <= 64 * 1024)
+      X.grep_log2 = lg;
   if (X.grep_log2 < 0) return mz_fail(MZ_ERR_CAPACITY, "adjoint workgroup would exceed 64 KB of LDS");
   // Lane replicas of the radial interval records reuse the partner-data area.
   X.nrad = c->meam ? X.gco0 : c->ncoef;
