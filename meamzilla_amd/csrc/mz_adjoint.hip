@@ -296,8 +296,8 @@ __global__ void __launch_bounds__(kAdjBlock) k_adjoint(const AdjArgs A) {
       Sa += push[1][q];
       push[1][q] = 0.0;
     }
-  Sa += __shfl_xor(Sa, 1, 4);
-  Sa += __shfl_xor(Sa, 2, 4);
+  Sa += dpp_move<kDppXor1>(Sa);
+  Sa += dpp_move<kDppXor2>(Sa);
   if (ba < ch.na && (tid & 3) == 0) {
     const int a = ba;
     const double rho = rho_b;

@@ -107,6 +107,27 @@ __device__ __forceinline__ bool eval_radial(const SplHead& H, const SplInterval*
   return eval_knot<Q>(H, IV + H.coef_off, r, v, d);
 }
 
+// Lane moves inside a row of sixteen lanes as DPP operand modes (VALU, no LDS
+// traffic): quad_perm [1,0,3,2] and [2,3,0,1] exchange lanes i and i^1, i and
+// i^2; row_half_mirror maps lane i of each group of eight to 7 - i; row_ror:8
+// exchanges lanes i and i^8.
+constexpr int kDppXor1 = 0xB1, kDppXor2 = 0x4E, kDppHalfMirror = 0x141, kDppRor8 = 0x128;
+
+template <int CTRL>
+__device__ __forceinline__ double dpp_move(double x) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), CTRL, 0xf, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
+
+// Sum over each aligned group of eight lanes, the same in every lane of the
+// group (pairs, then quads, then the two quads); all eight lanes must be active.
+__device__ __forceinline__ double group8_sum(double x) {
+  x += dpp_move<kDppXor1>(x);
+  x += dpp_move<kDppXor2>(x);
+  return x + dpp_move<kDppHalfMirror>(x);
+}
+
 // Sums three per-pair values (vals[q] for q in the atom's pair range) for
 // every atom of the chunk, eight lanes per atom, fixed order; out(a, x, y, z)
 // receives the sums.  Every lane of the workgroup must call it.
@@ -125,19 +146,16 @@ __device__ __forceinline__ void atom_sums3(const int* poff, int na, Get get, Out
         z += vz;
       }
     }
-#pragma unroll
-    for (int o = 4; o > 0; o >>= 1) {
-      x += __shfl_xor(x, o, 8);
-      y += __shfl_xor(y, o, 8);
-      z += __shfl_xor(z, o, 8);
-    }
+    x = group8_sum(x);
+    y = group8_sum(y);
+    z = group8_sum(z);
     if (a < na && l == 0) out(a, x, y, z);
   }
 }
 
-// Wave totals of eight per-lane values with ten shuffles: three halving
-// exchanges leave each lane with one value index summed over eight lanes, three
-// more steps finish the sum.  Lane l with (l & 7) == 0 returns the total of
+// Wave totals of eight per-lane values: three halving exchanges (the first
+// two across rows by shuffles, the third by DPP) leave each lane with one
+// value index summed over eight lanes; an eight-lane DPP sum finishes it.  Lane l with (l & 7) == 0 returns the total of
 // value index ((l >> 5) & 1) * 4 + ((l >> 4) & 1) * 2 + ((l >> 3) & 1).
 __device__ __forceinline__ double wave_sums8(const double (&v)[8]) {
   const int lane = threadIdx.x & 63;
@@ -155,11 +173,8 @@ __device__ __forceinline__ double wave_sums8(const double (&v)[8]) {
     const double keep = h4 ? a[i + 2] : a[i];
     b[i] = keep + __shfl_xor(send, 16, 64);
   }
-  double c = (h3 ? b[1] : b[0]) + __shfl_xor(h3 ? b[0] : b[1], 8, 64);
-  c += __shfl_xor(c, 4, 64);
-  c += __shfl_xor(c, 2, 64);
-  c += __shfl_xor(c, 1, 64);
-  return c;
+  const double c = (h3 ? b[1] : b[0]) + dpp_move<kDppRor8>(h3 ? b[0] : b[1]);
+  return group8_sum(c);
 }
 
 }  // namespace mz

@@ -654,12 +654,10 @@ __global__ void __launch_bounds__(BLOCK) k_resid(const ResidArgs a) {
       sy += t[1];
       sz += t[2];
     }
-#pragma unroll
-    for (int o = kResidLanes / 2; o > 0; o >>= 1) {
-      sx += __shfl_xor(sx, o, kResidLanes);
-      sy += __shfl_xor(sy, o, kResidLanes);
-      sz += __shfl_xor(sz, o, kResidLanes);
-    }
+    static_assert(kResidLanes == 8, "incoming-pair sums use eight-lane groups");
+    sx = group8_sum(sx);
+    sy = group8_sum(sy);
+    sz = group8_sum(sz);
     if (l < 3) {
       const int c = a.cell_of[i];
       const double w = a.weight[c];
