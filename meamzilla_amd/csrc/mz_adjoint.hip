@@ -50,7 +50,7 @@ struct AdjArgs {
   int natoms, ncells;
   long long npairs;
   const double* rho;       // [cand][natoms] unclamped densities
-  const double* G;         // [cand][npairs*3]
+  const double4* G;        // [cand][npairs]: three-body leg sum G_p and S1_p = sum_k g f_k
   const double* lam;       // [cand][natoms*3]
   const double* cell_adj;  // [cand][ncells*8]
   double* slots;           // [cand][kSlots][ntab]
@@ -191,15 +191,16 @@ __global__ void __launch_bounds__(kAdjBlock) k_adjoint(const AdjArgs A) {
   // leg sums, then positions, types and force duals of both ends.
   uint2 pr = make_uint2(0u, 0u);
   int ig = ch.a0;
-  double G0 = 0, G1 = 0, G2 = 0;
+  double G0 = 0, G1 = 0, G2 = 0, S1 = 0;
   if (has) {
     pr = A.pairs[ch.p0 + p];
     ig = A.owner[ch.p0 + p];
     if (MEAM) {
-      const double* G = A.G + ((size_t)cand * A.npairs + ch.p0 + p) * 3;
-      G0 = G[0];
-      G1 = G[1];
-      G2 = G[2];
+      const double4 Gq = A.G[(size_t)cand * A.npairs + ch.p0 + p];
+      G0 = Gq.x;
+      G1 = Gq.y;
+      G2 = Gq.z;
+      S1 = Gq.w;
     }
   }
   double rho_b = 0.0;
@@ -321,8 +322,8 @@ __global__ void __launch_bounds__(kAdjBlock) k_adjoint(const AdjArgs A) {
   // ---- phase C: unordered triplets {j, k} of each atom.
   // With B_jk = (w_j . d_k - a_j cos) / r_j and X = psi g + U' g' (B_jk + B_kj),
   // one visit adds f_k X + U' g A_k to the f-value adjoint of pair j and
-  // f_j X + U' g A_j to that of pair k (A = a f'), g f_k and g f_j to their
-  // f'-adjoint sums, and the g-coefficient adjoint of weights
+  // f_j X + U' g A_j to that of pair k (A = a f'), and the g-coefficient
+  // adjoint of weights
   // psi f_j f_k + U' (A_j f_k + A_k f_j) (value) and U' f_j f_k (B_jk + B_kj)
   // (slope).  The lane of pair j visits partners j+1 .. j+(n-1)/2 (mod n),
   // plus j+n/2 for the first n/2 lanes when n is even (the forward pass's
@@ -344,7 +345,7 @@ __global__ void __launch_bounds__(kAdjBlock) k_adjoint(const AdjArgs A) {
     double* gacl = gacc + grec * NC * R + (tid & (R - 1));
     double* ghead = atab + ncoef * 6 + sg * 6;
     const double Aj = aj * dfj, Wjx = invr * wx, Wjy = invr * wy, Wjz = invr * wz, Cj = invr * aj;
-    double Rj = 0.0, Qj = 0.0;
+    double Rj = 0.0;
     int kl = l;
     for (int m = 0; m < trips; ++m) {
       kl = (kl + 1 == n) ? 0 : kl + 1;
@@ -393,9 +394,7 @@ __global__ void __launch_bounds__(kAdjBlock) k_adjoint(const AdjArgs A) {
       const double X = ps * gv + up * gp * Bs;
       const double ff = fj * fk;
       Rj += fk * X + up * gv * Ak;
-      Qj += gv * fk;
       atomicAdd(&push[0][k], fj * X + up * gv * Aj);
-      atomicAdd(&push[1][k], gv * fj);
       const double al = ps * ff + up * (Aj * fk + Ak * fj);
       const double be = up * ff * Bs;
       if (POINT && extrap) {
@@ -420,7 +419,6 @@ __global__ void __launch_bounds__(kAdjBlock) k_adjoint(const AdjArgs A) {
       }
     }
     Phi = Rj;
-    Phip = Qj;
   }
   // ---- phase D: pair functions.  Pairs of similar length hit the same
   // interval, so their coefficient adjoints go to RR lane replicas of the
@@ -433,7 +431,7 @@ __global__ void __launch_bounds__(kAdjBlock) k_adjoint(const AdjArgs A) {
   for (int q = tid; q < RS * RR; q += kAdjBlock) rrep[q] = 0.0;
   if (MEAM && has) {
     Phi += push[0][p];
-    Phip = Up[ia] * aj * (Phip + push[1][p]);
+    Phip = Up[ia] * aj * S1;
   }
   __syncthreads();
   if (has) {
@@ -761,7 +759,7 @@ int ensure_adjoint_buffers(mz_ctx* c, int ncand) {
   mz::AdjointExtra& X = c->adjx;
   if (ncand <= X.cap) return MZ_OK;
   const size_t nc = (size_t)ncand;
-  bool bad = X.d_rho.alloc(nc * c->natoms) || X.d_dF.alloc(nc * c->natoms) || X.d_G.alloc(nc * c->npairs * 3) ||
+  bool bad = X.d_rho.alloc(nc * c->natoms) || X.d_dF.alloc(nc * c->natoms) || X.d_G.alloc(nc * c->npairs * 4) ||
              X.d_lam.alloc(nc * c->natoms * 3) || X.d_cadj.alloc(nc * c->ncells * 8) ||
              X.d_slots.alloc(nc * mz::kSlots * X.ntab) || X.d_grad.alloc(nc * c->nknot);
   if (bad) return mz_fail(MZ_ERR_NOMEM, "device allocation of gradient buffers failed");
@@ -818,7 +816,7 @@ static int run_grad(mz_ctx* c, int ncand) {
   a.ncells = c->ncells;
   a.npairs = c->npairs;
   a.rho = X.d_rho.p;
-  a.G = X.d_G.p;
+  a.G = reinterpret_cast<const double4*>(X.d_G.p);
   a.lam = X.d_lam.p;
   a.cell_adj = X.d_cadj.p;
   a.slots = X.d_slots.p;
@@ -877,7 +875,7 @@ int mz_grad_impl(mz_ctx* c, int ncand, const double* knots, double* F, double* G
   if (rc) return rc;
   mz::AdjointExtra& X = c->adjx;
   // bound the memory of the stored leg vectors: sub-batches of candidates
-  const size_t per_cand = (size_t)c->npairs * 6 * sizeof(double) + (size_t)c->natoms * 120 + (size_t)c->ncells * 64 +
+  const size_t per_cand = (size_t)c->npairs * 7 * sizeof(double) + (size_t)c->natoms * 120 + (size_t)c->ncells * 64 +
                           c->chunks.size() * 64 + (size_t)mz::kSlots * X.ntab * 8;
   int sub = (int)std::max<size_t>(1, std::min<size_t>((size_t)ncand, (size_t)(48ull << 30) / std::max<size_t>(per_cand, 1)));
   mz_timing acc{};  // kernel times summed over the sub-batches of a timed call

@@ -40,7 +40,8 @@ struct AtomArgs {
   double* chunk_acc;      // [ncand][nchunks][8]: E, S(6), constraint of each chunk
   double* rho_out;        // optional [ncand][natoms] density before clamping
   double* dF_out;         // optional [ncand][natoms] U'(rho)
-  double* G_out;          // optional [ncand][npairs*3] dF-free three-body leg vectors
+  double* G_out;          // optional [ncand][npairs][4]: dF-free three-body leg vector G_p and
+                          // S1_p = sum_k g f_k (the f'-adjoint factor of the reverse sweep)
 };
 
 struct ResidArgs {

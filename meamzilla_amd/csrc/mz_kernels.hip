@@ -375,7 +375,8 @@ __global__ void __launch_bounds__(BLOCK) k_atoms(const AtomArgs args) {
   // per-lane pair state
   int ia[PPL];
   double r[PPL], invr[PPL], ux[PPL], uy[PPL], uz[PPL];
-  double aphi[PPL][CT], brho[PPL][CT], fv[PPL][CT], dfv[PPL][CT], Gx[PPL][CT], Gy[PPL][CT], Gz[PPL][CT];
+  double aphi[PPL][CT], brho[PPL][CT], fv[PPL][CT], dfv[PPL][CT], Gx[PPL][CT], Gy[PPL][CT], Gz[PPL][CT],
+      S1v[PPL][CT];
   double e_acc[CT];
 #pragma unroll
   for (int c = 0; c < CT; ++c) e_acc[c] = 0.0;
@@ -388,7 +389,7 @@ __global__ void __launch_bounds__(BLOCK) k_atoms(const AtomArgs args) {
 #pragma unroll
     for (int c = 0; c < CT; ++c) {
       aphi[s][c] = brho[s][c] = fv[s][c] = dfv[s][c] = 0.0;
-      Gx[s][c] = Gy[s][c] = Gz[s][c] = 0.0;
+      Gx[s][c] = Gy[s][c] = Gz[s][c] = S1v[s][c] = 0.0;
     }
     if (p < ch.np) {
       int lo = 0, hi = ch.na - 1;
@@ -506,6 +507,7 @@ __global__ void __launch_bounds__(BLOCK) k_atoms(const AtomArgs args) {
         Gx[s][c] = xj * radial + fr * S[c][2];
         Gy[s][c] = yj * radial + fr * S[c][3];
         Gz[s][c] = zj * radial + fr * S[c][4];
+        S1v[s][c] = S[c][0];
       }
     }
   }
@@ -573,12 +575,8 @@ __global__ void __launch_bounds__(BLOCK) k_atoms(const AtomArgs args) {
       const double tz = uz[s] * sc + dFi * Gz[s][c];
       const size_t pg = (size_t)cand[c] * args.npairs + ch.p0 + p;
       if (valid[c]) {
-        if (args.G_out) {
-          double* g = args.G_out + pg * 3;
-          g[0] = Gx[s][c];
-          g[1] = Gy[s][c];
-          g[2] = Gz[s][c];
-        }
+        if (args.G_out)
+          reinterpret_cast<double4*>(args.G_out)[pg] = make_double4(Gx[s][c], Gy[s][c], Gz[s][c], S1v[s][c]);
         double* tv = args.tvec + pg * 3;
         tv[0] = tx;
         tv[1] = ty;
