@@ -81,9 +81,10 @@ struct ChainArgs {
 };
 
 // Partner data of every pair, structure of arrays (consecutive partners of
-// consecutive lanes read consecutive doubles): unit vector (3), f, a f',
-// w / r (3), a / r.
-constexpr int kPartnerFields = 9;
+// consecutive lanes read consecutive doubles): unit vector d (3), f, a f',
+// V = (w - a d) / r (3), the part of the dual normal to the pair over its
+// length, with which B_kj = (w_k . d_j - a_k cos) / r_k = V_k . d_j.
+constexpr int kPartnerFields = 8;
 
 // Where interval-coefficient adjoints go: coefficient m of interval record rec
 // at base[rec * rs + m * cs] (the workgroup table: rs 6, cs 1; a lane replica
@@ -279,7 +280,8 @@ __global__ void __launch_bounds__(kAdjBlock) k_adjoint(const AdjArgs A) {
     double sv = aj * drho;
     if (MEAM) {
       sv += wx * G0 + wy * G1 + wz * G2;
-      const double pv[kPartnerFields] = {ux, uy, uz, fj, aj * dfj, invr * wx, invr * wy, invr * wz, invr * aj};
+      const double pv[kPartnerFields] = {ux,      uy, uz, fj, aj * dfj, invr * (wx - aj * ux), invr * (wy - aj * uy),
+                                         invr * (wz - aj * uz)};
 #pragma unroll
       for (int q = 0; q < kPartnerFields; ++q) pd[q][p] = pv[q];
       push[0][p] = 0.0;
@@ -320,7 +322,8 @@ __global__ void __launch_bounds__(kAdjBlock) k_adjoint(const AdjArgs A) {
   __syncthreads();
 
   // ---- phase C: unordered triplets {j, k} of each atom.
-  // With B_jk = (w_j . d_k - a_j cos) / r_j and X = psi g + U' g' (B_jk + B_kj),
+  // With B_jk = (w_j . d_k - a_j cos) / r_j = V_j . d_k and
+  // X = psi g + U' g' (B_jk + B_kj),
   // one visit adds f_k X + U' g A_k to the f-value adjoint of pair j and
   // f_j X + U' g A_j to that of pair k (A = a f'), and the g-coefficient
   // adjoint of weights
@@ -344,7 +347,8 @@ __global__ void __launch_bounds__(kAdjBlock) k_adjoint(const AdjArgs A) {
     const double2* gtg = reinterpret_cast<const double2*>(gt + grec * NC);
     double* gacl = gacc + grec * NC * R + (tid & (R - 1));
     double* ghead = atab + ncoef * 6 + sg * 6;
-    const double Aj = aj * dfj, Wjx = invr * wx, Wjy = invr * wy, Wjz = invr * wz, Cj = invr * aj;
+    const double Aj = aj * dfj;
+    const double Vjx = invr * (wx - aj * ux), Vjy = invr * (wy - aj * uy), Vjz = invr * (wz - aj * uz);
     double Rj = 0.0;
     int kl = l;
     for (int m = 0; m < trips; ++m) {
@@ -352,7 +356,6 @@ __global__ void __launch_bounds__(kAdjBlock) k_adjoint(const AdjArgs A) {
       const int k = kb + kl;
       const double4 Pa = make_double4(pd[0][k], pd[1][k], pd[2][k], pd[3][k]);
       const double4 Pb = make_double4(pd[4][k], pd[5][k], pd[6][k], pd[7][k]);
-      const double Ck = pd[8][k];
       const double cs = ux * Pa.x + uy * Pa.y + uz * Pa.z;
       double gv, dgt, t;
       int fl;
@@ -390,7 +393,7 @@ __global__ void __launch_bounds__(kAdjBlock) k_adjoint(const AdjArgs A) {
       const double fk = keep ? Pa.w : 0.0;
       const double Ak = keep ? Pb.x : 0.0;
       const double gp = dgt * ginv;
-      const double Bs = (Wjx * Pa.x + Wjy * Pa.y + Wjz * Pa.z - Cj * cs) + (Pb.y * ux + Pb.z * uy + Pb.w * uz - Ck * cs);
+      const double Bs = (Vjx * Pa.x + Vjy * Pa.y + Vjz * Pa.z) + (Pb.y * ux + Pb.z * uy + Pb.w * uz);
       const double X = ps * gv + up * gp * Bs;
       const double ff = fj * fk;
       Rj += fk * X + up * gv * Ak;
