@@ -96,6 +96,7 @@ void launch_zero(hipStream_t st, double* p, size_t n);
 // CT candidates per workgroup (1, 2 or 4); returns -1 for an unsupported shape.
 int launch_atoms(hipStream_t st, const AtomArgs& a, int nchunks, int ncand, int block, int ppl, int ct);
 // Dynamic (spline tables) and static LDS bytes of one atom workgroup.
+bool atoms_tables_global(const PotDesc& pot, int ct);  // k_atoms reads spline tables from global memory
 size_t atoms_dyn_lds(const PotDesc& pot, int ct);
 size_t atoms_static_lds(int block, int ppl, int ct);
 int resid_blocks(int natoms, int ncells);

@@ -325,8 +325,9 @@ struct AtomSmem {
   double red8[CT][8][BLOCK / 64];
 };
 
-// One workgroup per (chunk, group of CT candidates).
-template <int BLOCK, int PPL, int MAXA, int CT, bool MEAM, bool POINT, bool Q>
+// One workgroup per (chunk, group of CT candidates).  TG: the spline tables
+// are read from global memory instead of being staged in LDS (large tables).
+template <int BLOCK, int PPL, int MAXA, int CT, bool MEAM, bool POINT, bool Q, bool TG>
 __global__ void __launch_bounds__(BLOCK) k_atoms(const AtomArgs args) {
   extern __shared__ __attribute__((aligned(64))) char dyn[];
   __shared__ __attribute__((aligned(16))) AtomSmem<BLOCK, PPL, MAXA, CT> sm;
@@ -343,27 +344,42 @@ __global__ void __launch_bounds__(BLOCK) k_atoms(const AtomArgs args) {
     cand[c] = valid[c] ? cc : args.ncand - 1;
   }
 
-  // stage the candidates' spline tables in LDS, then their g intervals again
-  // as compact power-of-two records (c0..c3, or c0..c5 padded to 8 doubles)
+  // LDS per candidate: its spline table (unless TG; then the table is read
+  // from global memory, cache-resident, per pair and per atom), the g splines'
+  // heads (TG only; otherwise they are part of the staged table), then the g
+  // intervals again as compact power-of-two records (c0..c3, or c0..c5 padded
+  // to 8 doubles), which the triplet walk reads per triplet.
   const int gbytes = MEAM ? args.pot.ngco * NG * 8 : 0;
+  const int ghb = MEAM && TG ? args.pot.nt * (int)sizeof(SplHead) : 0;
+  const int tbytes = TG ? 0 : TB;
+  const int cbytes = tbytes + ghb + gbytes;
 #pragma unroll
   for (int c = 0; c < CT; ++c) {
-    const int4* src = reinterpret_cast<const int4*>(args.tables + (size_t)cand[c] * TB);
-    int4* dst = reinterpret_cast<int4*>(dyn + (size_t)c * TB);
-    for (int q = tid; q < TB / 16; q += BLOCK) dst[q] = src[q];
+    const char* tb = args.tables + (size_t)cand[c] * TB;
+    char* cb = dyn + (size_t)c * cbytes;
+    if (!TG) {
+      const int4* src = reinterpret_cast<const int4*>(tb);
+      for (int q = tid; q < TB / 16; q += BLOCK) reinterpret_cast<int4*>(cb)[q] = src[q];
+    }
     if (MEAM) {
-      const SplInterval* gsrc = reinterpret_cast<const SplInterval*>(args.tables + (size_t)cand[c] * TB +
-                                                                     sizeof(SplHead) * args.pot.nspl);
-      double* gdst = reinterpret_cast<double*>(dyn + (size_t)CT * TB + (size_t)c * gbytes);
+      if (TG) {
+        const int4* hsrc = reinterpret_cast<const int4*>(tb + sizeof(SplHead) * args.pot.fn_off[4]);
+        for (int q = tid; q < ghb / 16; q += BLOCK) reinterpret_cast<int4*>(cb)[q] = hsrc[q];
+      }
+      const SplInterval* gsrc = reinterpret_cast<const SplInterval*>(tb + sizeof(SplHead) * args.pot.nspl);
+      double* gdst = reinterpret_cast<double*>(cb + tbytes + ghb);
       for (int q = tid; q < args.pot.ngco * NG; q += BLOCK)
         gdst[q] = (q % NG) < 6 ? gsrc[args.pot.gco0 + q / NG].c[q % NG] : 0.0;
     }
   }
-  auto HDc = [&](int c) { return reinterpret_cast<const SplHead*>(dyn + (size_t)c * TB); };
-  auto IVc = [&](int c) {
-    return reinterpret_cast<const SplInterval*>(dyn + (size_t)c * TB + sizeof(SplHead) * args.pot.nspl);
+  auto HDc = [&](int c) {
+    return reinterpret_cast<const SplHead*>(TG ? args.tables + (size_t)cand[c] * TB : dyn + (size_t)c * cbytes);
   };
-  auto GTc = [&](int c) { return reinterpret_cast<const double*>(dyn + (size_t)CT * TB + (size_t)c * gbytes); };
+  auto IVc = [&](int c) { return reinterpret_cast<const SplInterval*>(HDc(c) + args.pot.nspl); };
+  auto GHc = [&](int c) {  // g heads by element type
+    return TG ? reinterpret_cast<const SplHead*>(dyn + (size_t)c * cbytes) : HDc(c) + args.pot.fn_off[4];
+  };
+  auto GTc = [&](int c) { return reinterpret_cast<const double*>(dyn + (size_t)c * cbytes + tbytes + ghb); };
   for (int a = tid; a <= ch.na; a += BLOCK) sm.poff[a] = args.atom_poff[ch.a0 + a] - ch.p0;
   __syncthreads();
 
@@ -473,7 +489,8 @@ __global__ void __launch_bounds__(BLOCK) k_atoms(const AtomArgs args) {
       const int p = tid + s * BLOCK;
       if (ia[s] < 0) continue;
       const int kb = sm.poff[ia[s]], ke = sm.poff[ia[s] + 1];
-      const SplHead& Hg = HD0[args.pot.fn_off[4] + args.types[ch.a0 + ia[s]]];
+      const int ti = args.types[ch.a0 + ia[s]];
+      const SplHead& Hg = GHc(0)[ti];
       const int goff = (Hg.coef_off - args.pot.gco0) * NG;
       const double xj = ux[s], yj = uy[s], zj = uz[s];
       double S[CT][5];
@@ -489,7 +506,7 @@ __global__ void __launch_bounds__(BLOCK) k_atoms(const AtomArgs args) {
 #pragma unroll
         for (int c = 0; c < CT; ++c) {
           double gv, gp;
-          g_value<POINT, Q>(HDc(c)[args.pot.fn_off[4] + args.types[ch.a0 + ia[s]]], GTc(c) + goff, L, gv, gp);
+          g_value<POINT, Q>(GHc(c)[ti], GTc(c) + goff, L, gv, gp);
           const double fk = zero ? 0.0 : (c == 0 ? P.w : sm.fx[c > 0 ? c - 1 : 0][k]);
           const double b = gp * fk;
           S[c][0] += gv * fk;
@@ -768,22 +785,37 @@ static void launch_atoms_t(hipStream_t st, const AtomArgs& a, int nchunks, int n
   const bool point = style == kEam || style == kMeam;
   const bool q = a.pot.quintic != 0;
   const size_t dyn = atoms_dyn_lds(a.pot, CT);
+  const bool tg = atoms_tables_global(a.pot, CT);
   dim3 grid(nchunks, (ncand + CT - 1) / CT), block(BLOCK);
-#define MZ_LAUNCH(M, P, Q_)                                                                          \
-  hipLaunchKernelGGL((k_atoms<BLOCK, PPL, MAXA, CT, M, P, Q_>), grid, block, dyn, st, a)
-  if (meam) {
-    if (point) { if (q) MZ_LAUNCH(true, true, true); else MZ_LAUNCH(true, true, false); }
-    else { if (q) MZ_LAUNCH(true, false, true); else MZ_LAUNCH(true, false, false); }
+#define MZ_LAUNCH(M, P, Q_, TG_)                                                                    \
+  hipLaunchKernelGGL((k_atoms<BLOCK, PPL, MAXA, CT, M, P, Q_, TG_>), grid, block, dyn, st, a)
+  if (meam && tg) {
+    if (point) { if (q) MZ_LAUNCH(true, true, true, true); else MZ_LAUNCH(true, true, false, true); }
+    else { if (q) MZ_LAUNCH(true, false, true, true); else MZ_LAUNCH(true, false, false, true); }
+  } else if (meam) {
+    if (point) { if (q) MZ_LAUNCH(true, true, true, false); else MZ_LAUNCH(true, true, false, false); }
+    else { if (q) MZ_LAUNCH(true, false, true, false); else MZ_LAUNCH(true, false, false, false); }
   } else {
-    if (point) { if (q) MZ_LAUNCH(false, true, true); else MZ_LAUNCH(false, true, false); }
-    else { if (q) MZ_LAUNCH(false, false, true); else MZ_LAUNCH(false, false, false); }
+    if (point) { if (q) MZ_LAUNCH(false, true, true, false); else MZ_LAUNCH(false, true, false, false); }
+    else { if (q) MZ_LAUNCH(false, false, true, false); else MZ_LAUNCH(false, false, false, false); }
   }
 #undef MZ_LAUNCH
 }
 
+// MEAM potentials whose staged tables would take more than 16 KB of LDS per
+// workgroup read them from global memory instead (measured: the ternary
+// potential's 17.5 KB tables cost a candidate per workgroup and a workgroup
+// per CU; the one-element potential's 3.9 KB are faster staged).
+bool atoms_tables_global(const PotDesc& pot, int ct) {
+  const bool meam = pot.style == kMeamSpline || pot.style == kMeam;
+  return meam && (size_t)ct * (size_t)pot.table_bytes > 16 * 1024;
+}
+
 size_t atoms_dyn_lds(const PotDesc& pot, int ct) {
   const bool meam = pot.style == kMeamSpline || pot.style == kMeam;
-  return (size_t)ct * ((size_t)pot.table_bytes + (meam ? (size_t)pot.ngco * 8 * g_record_doubles(pot.quintic != 0) : 0));
+  const size_t g = meam ? (size_t)pot.ngco * 8 * g_record_doubles(pot.quintic != 0) : 0;
+  if (atoms_tables_global(pot, ct)) return (size_t)ct * ((size_t)pot.nt * sizeof(SplHead) + g);
+  return (size_t)ct * ((size_t)pot.table_bytes + g);
 }
 
 size_t atoms_static_lds(int block, int ppl, int ct) {
