@@ -86,6 +86,23 @@ def test_adjoint_matches_oracle_cd(cfg, scale):
     assert _rel(g[slots], gcd) < GTOL
 
 
+def test_adjoint_ternary_matches_central_differences():
+    """Ternary potential (three g splines, eight g-adjoint replicas, forward
+    tables read from global memory): the adjoint against central differences
+    of the GPU objective on a seeded sample of the 252 coefficients."""
+    _say("ternary adjoint vs CD")
+    w = gen.workload(4, scale=2 / 2000)
+    gen.add_targets(w, pyoracle.evaluator)
+    obj = Objective(w.pot, w.db)
+    kv = w.pot.knot_vector()
+    F, g = obj.objective_grad(kv)
+    assert abs(F - obj.objective(kv)) <= 1e-12 * abs(F)
+    slots = np.asarray(w.pot.coeff_slots())
+    idx = np.sort(np.random.default_rng(7).choice(slots, size=48, replace=False))
+    gcd = _cd(obj.objective, kv, idx)
+    assert _rel(g[idx], gcd) < GTOL, _rel(g[idx], gcd)
+
+
 def test_batched_gradient_matches_single():
     _say("batched gradient")
     w = gen.workload(2, scale=12 / 512)

@@ -31,7 +31,7 @@ def test_gpu_matches_reference_golden(case):
     assert rel_err(obj.densities(), exp["densities"]) < TOL
 
 
-@pytest.mark.parametrize("cfg,scale", [(0, 1.0), (1, 24 / 256), (2, 40 / 512), (3, 30 / 1000)])
+@pytest.mark.parametrize("cfg,scale", [(0, 1.0), (1, 24 / 256), (2, 40 / 512), (3, 30 / 1000), (4, 3 / 2000)])
 def test_gpu_matches_oracle_workloads(cfg, scale):
     _say("workload", cfg, scale)
     w = gen.workload(cfg, scale=scale)
