@@ -158,6 +158,19 @@ typedef struct {
 int mz_enable_timing(mz_ctx* ctx, int on);
 int mz_last_timing(const mz_ctx* ctx, mz_timing* out);
 
+/* Second derivatives y'' (and, for quintic splines, fourth derivatives y'''')
+ * at the knots of one spline from its knot record [y_0..y_{n-1}, grad_L,
+ * grad_R, grad2_L, grad2_R], computed on the host with the same algorithms as
+ * the device refresh.  Replaces Spline3Eq::refresh_basis
+ * (src/spline3eq.cpp:192-225) and Spline5Eq::refresh_basis
+ * (src/spline5eq.cpp:210-309) for callers that print splines, such as the
+ * LAMMPS export (src/spline3eq.cpp:279-295, src/spline5eq.cpp:358-377).
+ * basis is 3 or 5, 3 <= n <= 32, x0 < xn; natural_left / natural_right are 1
+ * for a natural end.  ypp and yp4 hold n doubles (yp4 may be NULL for basis 3,
+ * whose y'''' are zero).  No device is used. */
+int mz_spline_refresh(int basis, int n, double x0, double xn, int natural_left, int natural_right,
+                      const double* record, double* ypp, double* yp4);
+
 #ifdef __cplusplus
 }
 #endif

@@ -85,8 +85,19 @@ def lib() -> C.CDLL:
     L.mz_autotune.argtypes = [vp, C.c_int, C.c_int, C.POINTER(Tile)]
     L.mz_enable_timing.argtypes = [vp, C.c_int]
     L.mz_last_timing.argtypes = [vp, C.POINTER(Timing)]
+    L.mz_spline_refresh.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int, dp, dp, dp]
     _lib = L
     return L
+
+
+def spline_refresh(spline) -> tuple:
+    """y'' and y'''' at the knots of one spline (mz_spline_refresh, host only)."""
+    rec = np.ascontiguousarray(spline.knot_record(), dtype=np.float64)
+    ypp = np.zeros(spline.n)
+    yp4 = np.zeros(spline.n)
+    _check(lib().mz_spline_refresh(int(spline.basis), int(spline.n), float(spline.x0), float(spline.xn),
+                                   int(spline.bc[0] == -1), int(spline.bc[1] == -1), _dp(rec), _dp(ypp), _dp(yp4)))
+    return ypp, yp4
 
 
 def _check(rc: int) -> None:

@@ -959,3 +959,14 @@ void mz_adjoint_drop(mz_ctx* c) {
 extern "C" int mz_objective_grad(mz_ctx* c, const double* knots, double* F, double* dF) {
   return mz_grad_impl(c, 1, knots, F, dF);
 }
+
+extern "C" int mz_spline_refresh(int basis, int n, double x0, double xn, int natural_left, int natural_right,
+                                 const double* record, double* ypp, double* yp4) {
+  if (!record || !ypp || (basis == 5 && !yp4)) return mz_fail(MZ_ERR_ARG, "null argument");
+  if ((basis != 3 && basis != 5) || n < 3 || n > mz::kMaxKnots || !(xn > x0))
+    return mz_fail(MZ_ERR_ARG, "unsupported spline (basis 3 or 5, 3 <= n <= 32, x0 < xn)");
+  std::vector<double> four(n, 0.0);
+  host_refresh(basis, n, x0, xn, natural_left != 0, natural_right != 0, record, ypp, yp4 ? yp4 : four.data());
+  return MZ_OK;
+}
+

@@ -355,6 +355,28 @@ double Potential::compute(ErrorVec* ev) {
   return F;
 }
 
+double Potential::compute_outputs(ErrorVec* ev, std::vector<double>* cell_energy, std::vector<double>* cell_stress,
+                                  std::vector<double>* forces) {
+  std::vector<double> kv = knot_vector();
+  const mz_info_t& in = ev_->info();
+  ev->assign(in.nerr, 0.0);
+  cell_energy->assign(in.ncells, 0.0);
+  cell_stress->assign(6 * (size_t)in.ncells, 0.0);
+  forces->assign(3 * (size_t)in.natoms, 0.0);
+  double F = 0.0;
+  check(mz_objective(ev_->ctx(), kv.data(), &F, ev->data(), cell_energy->data(), cell_stress->data(),
+                     forces->data()));
+  error_sum_ = F;
+  ++ncalls_;
+  return F;
+}
+
+void Potential::compute_densities(std::vector<double>* dens) {
+  std::vector<double> kv = knot_vector();
+  dens->assign(ev_->info().natoms, 0.0);
+  check(mz_densities(ev_->ctx(), kv.data(), dens->data()));
+}
+
 double Potential::compute_grad(std::vector<double>* grad) {
   std::vector<double> kv = knot_vector();
   std::vector<double> g(kv.size());
@@ -975,6 +997,189 @@ std::vector<Potential*> genalg(const std::vector<Potential*>& pop_in, const OptO
   return pop;
 }
 
+// ---------------------------------------------------------------- output files
+
+namespace {
+
+// dy/dx of a refreshed spline in interval k at shift t (a = 1 - t): the
+// derivative of the cubic (and quintic) interpolation formula of
+// src/spline3eq.h:96-105 and src/spline5eq.h:95-107.
+double spline_slope(const SplineRec& s, const std::vector<double>& ypp, const std::vector<double>& yp4, int k,
+                    double t) {
+  const double step = (s.xn - s.x0) / (s.n - 1), invstep = 1. / step, a = 1.0 - t;
+  double d = (s.y[k + 1] - s.y[k]) * invstep + ((3 * t * t - 1) * ypp[k + 1] - (3 * a * a - 1) * ypp[k]) * step / 6.0;
+  if (s.basis == 5)
+    d += ((15 * t * t * t * t - 30 * t * t + 7) * yp4[k + 1] - (15 * a * a * a * a - 30 * a * a + 7) * yp4[k]) *
+         (step * step * step) / 360.0;
+  return d;
+}
+
+std::string numbered(std::string base, const char* suffix, int num) {
+  base += suffix;
+  if (num >= 0) base += "." + std::to_string(num);
+  return base;
+}
+
+std::ofstream open_out(const std::string& path) {
+  std::ofstream out(path);
+  if (!out) throw Error("Unable to open file to write: " + path);
+  return out;
+}
+
+}  // namespace
+
+void write_lammps_file(const std::string& path, const PotFile& pf) {
+  for (size_t p = 0; p < pf.pots.size(); ++p) {
+    std::ofstream out = open_out(pf.pots.size() > 1 ? path + "." + std::to_string(p) : path);
+    out << pf.comment << "\n" << pf.style << "\n" << std::scientific << std::setprecision(15);
+    for (const SplineRec& s : pf.pots[p]) {
+      const int n = s.n;
+      std::vector<double> rec(s.y);
+      rec.insert(rec.end(), {s.grad[0], s.grad[1], s.grad2[0], s.grad2[1]});
+      std::vector<double> ypp(n), yp4(n);
+      check(mz_spline_refresh(s.basis, n, s.x0, s.xn, s.bc[0] == -1, s.bc[1] == -1, rec.data(), ypp.data(),
+                              yp4.data()));
+      out << (s.basis == 3 ? "spline3eq" : "spline5eq") << "\n" << n << "\n";
+      out << spline_slope(s, ypp, yp4, 0, 0.0) << " " << spline_slope(s, ypp, yp4, n - 2, 1.0) << "\n";
+      const double step = (s.xn - s.x0) / (n - 1);
+      for (int k = 0; k < n; ++k) {
+        const double x = k == 0 ? s.x0 : (k == n - 1 ? s.xn : s.x0 + step * k);
+        out << x << " " << s.y[k] << " " << ypp[k];
+        if (s.basis == 5) out << " " << yp4[k];
+        out << "\n";
+      }
+    }
+    out << "\n";
+  }
+}
+
+void write_data_files(const std::string& base, Potential& pot, const ConfigDb& db, int num) {
+  ErrorVec ev;
+  std::vector<double> ce, cs, fr, dens;
+  pot.compute_outputs(&ev, &ce, &cs, &fr);
+  pot.compute_densities(&dens);
+  const int nc = db.ncells;
+  int natoms = 0;
+  for (int c = 0; c < nc; ++c) natoms += db.natoms[c];
+  const bool meam = pot.style().compare(0, 4, "meam") == 0;
+  using std::setw;
+  using std::setprecision;
+  {  // densities (src/pots/pot_eam/pot_eam.cpp:506-543)
+    std::ofstream o = open_out(numbered(base, ".rho", num));
+    o << setw(6) << std::right << "conf:" << setw(5) << "atom" << setw(8) << "type" << setw(15) << "rho" << "\n";
+    for (int c = 0, a = 0; c < nc; ++c)
+      for (int i = 0; i < db.natoms[c]; ++i, ++a)
+        o << setw(5) << c << ":" << setw(5) << i << setw(8) << db.types[a] << setw(15) << std::fixed << dens[a]
+          << "\n";
+  }
+  {  // energies (src/potential.cpp:497-542)
+    std::ofstream o = open_out(numbered(base, ".energy", num));
+    o << "Global energy weight is " << std::fixed << db.energy_weight << "\n";
+    o << std::right << setw(3) << "#" << setw(11) << "conf_w" << setw(11) << "conf_ew" << setw(9) << "natoms"
+      << setw(15) << "energy error" << setw(18) << "e" << setw(18) << "e0" << setw(15) << "|e-e0|" << setw(15)
+      << "e-e0" << setw(15) << "de/e0" << "\n";
+    for (int c = 0, idx = 0; c < nc; ++c) {
+      const double e = ce[c], e0 = db.energy0[c];
+      o << setw(3) << c << std::fixed << setw(11) << setprecision(4) << db.weight[c] << setw(11) << db.eweight[c]
+        << setw(9) << db.natoms[c] << setw(15) << setprecision(6) << ev[idx] * ev[idx] << setw(18)
+        << setprecision(10) << e << setw(18) << e0 << setw(15) << setprecision(6) << std::abs(e - e0) << setw(15)
+        << e - e0;
+      if (e0 == 0) o << setw(15) << "inf";
+      else o << setw(15) << (e0 - e) / e0;
+      o << "\n";
+      idx += 7 + 3 * db.natoms[c];
+    }
+  }
+  {  // stresses (src/potential.cpp:548-591)
+    std::ofstream o = open_out(numbered(base, ".stress", num));
+    o << "Global stress weight is " << std::fixed << db.stress_weight << "\n";
+    o << std::right << setw(3) << "#" << setw(11) << "conf_w" << setw(11) << "conf_sw" << setw(15) << "stress error"
+      << setw(18) << "s" << setw(18) << "s0" << setw(15) << "ds/s0" << "\n";
+    for (int c = 0, idx = 0; c < nc; ++c) {
+      ++idx;
+      for (int d = 0; d < 6; ++d, ++idx) {
+        const double sv = cs[6 * c + d], s0 = db.stress0[6 * c + d];
+        o << setw(3) << c << std::fixed << setw(11) << setprecision(4) << db.weight[c] << setw(11) << db.sweight[c]
+          << setw(15) << setprecision(6) << ev[idx] * ev[idx] << setw(18) << setprecision(10) << sv << setw(18)
+          << s0;
+        if (s0 == 0) o << setw(15) << "inf";
+        else o << setw(15) << setprecision(6) << (s0 - sv) / s0;
+        o << "\n";
+      }
+      idx += 3 * db.natoms[c];
+    }
+  }
+  {  // forces (src/potential.cpp:597-641)
+    std::ofstream o = open_out(numbered(base, ".force", num));
+    o << std::right << setw(6) << "conf:" << setw(5) << "atom" << setw(8) << "type" << setw(15) << "force error"
+      << setw(15) << "f" << setw(15) << "f0" << setw(15) << "df/f0" << "\n";
+    for (int c = 0, idx = 0, a = 0; c < nc; ++c) {
+      idx += 7;
+      for (int i = 0; i < db.natoms[c]; ++i, ++a)
+        for (int d = 0; d < 3; ++d, ++idx) {
+          const double f = fr[3 * a + d], f0 = db.force0[3 * a + d];
+          o << setw(5) << c << ":" << setw(5) << i << setw(8) << db.types[a] << setw(15) << std::fixed
+            << ev[idx] * ev[idx] << setw(15) << f << setw(15) << f0;
+          if (f0 == 0) o << setw(15) << "inf";
+          else o << setw(15) << (f0 - f) / f0;
+          o << "\n";
+        }
+    }
+  }
+  {  // error sums (src/potential.cpp:647-707)
+    double se = 0, ss = 0, sf = 0, sp = 0;
+    size_t idx = 0;
+    for (int c = 0; c < nc; ++c) {
+      se += ev[idx] * ev[idx];
+      ++idx;
+      for (int d = 0; d < 6; ++d, ++idx) ss += ev[idx] * ev[idx];
+      for (int i = 0; i < 3 * db.natoms[c]; ++i, ++idx) sf += ev[idx] * ev[idx];
+    }
+    for (; idx < ev.size(); ++idx) sp += ev[idx] * ev[idx];
+    const long npun = (long)ev.size() - 3L * natoms - 7L * nc;
+    std::ofstream o = open_out(numbered(base, ".error", num));
+    o << "Total error sum " << std::fixed << se + ss + sf + sp << " count " << ev.size() << " (" << 3 * natoms
+      << " forces, " << 6 * nc << " stresses, " << nc << " energies";
+    if (npun == 0) o << ")\n";
+    else o << ", " << npun << " punishments)\n";
+    o << "Sum of force errors  = " << sf << "\n";
+    o << "Sum of energy errors = " << se << "\n";
+    o << "Sum of stress errors = " << ss << "\n";
+    if (npun) o << "Sum of punish errors = " << sp << "\n";
+  }
+  {  // constraints (src/pots/pot_eam/pot_eam.cpp:439-488, src/pots/pot_meam/pot_meam.cpp:444-490)
+    std::ofstream o = open_out(numbered(base, ".punish", num));
+    o << "Limiting constraints (atomic density lies inside F boundaries)\n";
+    o << std::right << setw(3) << "#" << setw(11) << "conf_w" << setw(15) << "punish error" << setw(15)
+      << "punishment" << "\n";
+    size_t idx = 7 * (size_t)nc + 3 * (size_t)natoms;
+    for (int c = 0; c < nc; ++c, ++idx)
+      o << setw(3) << c << std::fixed << setw(11) << setprecision(4) << db.weight[c] << setw(15) << setprecision(6)
+        << ev[idx] * ev[idx] << setw(15) << ev[idx] / db.weight[c] << "\n";
+    const int nt = pot.ntypes();
+    if (meam) {
+      // signed knot value of largest magnitude over the f splines (src/pot_fns.cpp:100-111)
+      const int f0 = nt * (nt + 1) / 2 + 2 * nt, nf = nt * (nt + 1) / 2;
+      double fmax = 0.0;
+      for (int q = f0; q < f0 + nf; ++q) {
+        double m = 0.0;
+        for (double y : pot.splines()[q].y)
+          if (std::abs(m) < std::abs(y)) m = y;
+        if (std::abs(fmax) < std::abs(m)) fmax = m;
+      }
+      o << "\nDummy constraints (|f_max(r)| = 1)\n";
+      o << setw(15) << "punish error" << setw(15) << "|f_max(r)|" << "\n";
+      o << std::fixed << setw(15) << setprecision(6) << ev[idx] * ev[idx] << setw(15) << std::abs(fmax) << "\n";
+    } else {
+      o << "\nDummy constraints (F'(n_avg) = 0)\n";
+      o << setw(7) << "element" << setw(15) << "punish error" << setw(15) << "F'(n_avg)" << "\n";
+      for (int t = 0; t < nt; ++t, ++idx)
+        o << setw(7) << t << std::fixed << setw(15) << setprecision(6) << ev[idx] * ev[idx] << setw(15)
+          << ev[idx] / 100.0 << "\n";  // DUMMY_WEIGHT, src/pots/pot_pair/pot_pair.h:39
+    }
+  }
+}
+
 // ---------------------------------------------------------------- driver
 
 int run_param_file(const std::string& param_path, std::ostream& out) {
@@ -1042,6 +1247,10 @@ int run_param_file(const std::string& param_path, std::ostream& out) {
   endf.pots.clear();
   for (auto* p : pots) endf.pots.push_back(p->splines());
   if (prm.has("endpot")) write_potential_file(path(prm.str("endpot")), endf);
+  if (prm.has("lammpsfile")) write_lammps_file(path(prm.str("lammpsfile")), endf);
+  if (prm.has("datafile"))
+    for (size_t i = 0; i < pots.size(); ++i)
+      write_data_files(path(prm.str("datafile")), *pots[i], db, pots.size() > 1 ? (int)i : -1);
   for (size_t i = 0; i < pots.size(); ++i)
     out << "P#" << i << " - Final total sum error = " << std::fixed << pots[i]->compute() << std::endl;
   for (auto* p : pots) delete p;

@@ -102,6 +102,10 @@ class Potential {
   double& coeff(int idx);                                    // src/potential.cpp:192-206
   void modify_coeff(int idx, double height, double width);   // src/potential.cpp:212-226, src/spline.cpp:267-296
   double compute(ErrorVec* error_vec = nullptr);             // src/potential.h:67
+  // compute plus per-cell energies (per atom), stresses (per volume) and forces
+  double compute_outputs(ErrorVec* ev, std::vector<double>* cell_energy, std::vector<double>* cell_stress,
+                         std::vector<double>* forces);
+  void compute_densities(std::vector<double>* dens);        // src/pots/pot_eam/pot_eam.cpp:395-433
   double compute_grad(std::vector<double>* grad);            // objective + d/dcoeff (adjoint)
   void set_dir(const std::vector<double>& dir, double x);    // src/potential.cpp:268-274
   double compute_dir(ErrorVec* ev, const std::vector<double>& dir, double x);  // :281-292
@@ -178,8 +182,18 @@ Potential* simann(const Potential& start, const OptOptions& opt, std::mt19937& r
 // generation (src/opt_genalg.cpp:105-309).
 std::vector<Potential*> genalg(const std::vector<Potential*>& pop, const OptOptions& opt, std::mt19937& rng);
 
+// LAMMPS export of every potential of a list (src/pot_list.cpp:333-361,
+// src/pot_fns.cpp:360-368): comment line, style, then per spline its basis
+// name, knot count, end slopes and knot lines x y y'' [y''''].
+void write_lammps_file(const std::string& path, const PotFile& pf);
+// Fitting data files <base>.rho/.energy/.stress/.force/.error/.punish[.num]
+// (src/potential.cpp:325-342,497-707, src/pots/pot_eam/pot_eam.cpp:439-543,
+// src/pots/pot_meam/pot_meam.cpp:444-490); num < 0 for a single potential.
+void write_data_files(const std::string& base, Potential& pot, const ConfigDb& db, int num);
+
 // Runs a parameter file the way `meamz -p <file>` does: initial objective,
-// optimizer list, end potential, final objective.  Returns 0 on success.
+// optimizer list, end potential, LAMMPS and data files, final objective.
+// Returns 0 on success.
 int run_param_file(const std::string& param_path, std::ostream& out);
 
 }  // namespace mzh

@@ -5,12 +5,15 @@
 // Potential::compute(const Comm&, ErrorVec*) (src/potential.h:67) and prints
 // the results with full double precision as JSON on rank 0.
 //
-//   meamz_ref -p <param file> [--fd <h>] [--time <n>] [--dens]
+//   meamz_ref -p <param file> [--fd <h>] [--time <n>] [--dens] [--write]
 //
 //   --fd h    also print the central-difference gradient of the objective over
 //             every adjustable coefficient, step h*max(|c|,1)
 //   --time n  time n objective evaluations (all ranks) and print evals/s
 //   --dens    also print per-atom densities (PotEAM::compute_densities)
+//   --write   afterwards write the LAMMPS file and the data files named by the
+//             parameter file's lammpsfile / datafile keys, as the program does
+//             at the end of a run (src/meamzilla.cpp:149-150)
 #include "meamzilla.h"
 #include "universe.h"
 #include "config.h"
@@ -48,13 +51,14 @@ int main(int argc, char** argv) {
   MPI_Init(&argc, &argv);
   double fd_h = 0.0;
   int ntime = 0;
-  bool dens = false;
+  bool dens = false, write = false;
   std::vector<char*> args;
   args.push_back(argv[0]);
   for (int i = 1; i < argc; ++i) {
     if (!std::strcmp(argv[i], "--fd") && i + 1 < argc) fd_h = std::atof(argv[++i]);
     else if (!std::strcmp(argv[i], "--time") && i + 1 < argc) ntime = std::atoi(argv[++i]);
     else if (!std::strcmp(argv[i], "--dens")) dens = true;
+    else if (!std::strcmp(argv[i], "--write")) write = true;
     else args.push_back(argv[i]);
   }
   int rank = 0;
@@ -119,6 +123,10 @@ int main(int argc, char** argv) {
     }
     if (rank == 0) std::printf("]}\n");
     std::fflush(stdout);
+    if (write) {
+      if (!plist.get_lmp_file().empty()) plist.write_lmps(plist.get_lmp_file());
+      if (!plist.get_dat_file().empty()) plist.write_data(plist.get_dat_file());
+    }
   }
   MPI_Finalize();
   return 0;
