@@ -37,6 +37,8 @@ extern "C" {
 #define MZ_STYLE_MEAM_SPLINE 1 /* "meam/spline" */
 #define MZ_STYLE_EAM 2         /* "eam"  (point-evaluated) */
 #define MZ_STYLE_MEAM 3        /* "meam" (point-evaluated) */
+#define MZ_STYLE_GMEAM_SPLINE 4 /* "gmeam/spline": g indexed by (type i, types j k) */
+#define MZ_STYLE_GMEAM 5       /* "gmeam" (point-evaluated) */
 
 /* Spline layout of one potential.  bc holds two ints per spline:
  * -1 natural, 0 free slope, 1 fixed slope (the "#B" constraint line). */

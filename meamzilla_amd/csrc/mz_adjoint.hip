@@ -65,6 +65,7 @@ struct AdjArgs {
 struct ChainArgs {
   int nspl, ncoef, ntab, nknot, meam, nt;
   int f0, nf, u0;
+  int fper;                // one f-max penalty per f spline (gmeam)
   const int* nknots;
   const int* basis;
   const int* koff;
@@ -166,9 +167,23 @@ __device__ __forceinline__ void acc_radial(double* tab, const CoefSink& sk, int 
   acc_interval<Q>(sk, H.coef_off + k, t, al, be * H.invstep);
 }
 
+// What the sweep needs of one g spline.
+struct GSpl {
+  double x0, inv, xcut, xn, kmax, y0, yn, gL, gR, g2L, g2R;
+  int rec;        // first interval record, counted from the first g record
+  bool quintic;
+};
+
+__device__ __forceinline__ GSpl gspl_of(const SplHead& H, int gco0) {
+  return GSpl{H.x0, H.invstep, H.xcut, H.xn, (double)(H.n - 2), H.y0, H.yn, H.gL, H.gR, H.g2L, H.g2R,
+              H.coef_off - gco0, H.basis == 5};
+}
+
 // GR: lane replicas of the g-coefficient adjoints (32 keeps the 32 lanes that
 // one LDS cycle serves on distinct replicas; fewer when LDS is short).
-template <bool MEAM, bool POINT, bool Q, int GR>
+// GP: the g spline of a triplet depends on both neighbours' types (gmeam),
+// chosen per triplet from heads staged in LDS.
+template <bool MEAM, bool POINT, bool Q, int GR, bool GP>
 __global__ void __launch_bounds__(kAdjBlock) k_adjoint(const AdjArgs A) {
   extern __shared__ __attribute__((aligned(64))) char dyn[];
   __shared__ double pd[kPartnerFields][kAdjBlock];
@@ -176,6 +191,7 @@ __global__ void __launch_bounds__(kAdjBlock) k_adjoint(const AdjArgs A) {
   __shared__ double psi[kAdjAtoms], Up[kAdjAtoms];
   static_assert(kAdjAtoms * 4 == kAdjBlock, "phase B takes four lanes per atom");
   __shared__ int poff[kAdjAtoms + 1];
+  __shared__ int ptyp[GP ? kAdjBlock : 1];  // neighbour type of every pair (GP)
   constexpr int NC = Q ? 6 : 4;  // coefficients per g interval that carry adjoints
   const int cand = blockIdx.y;
   const int tid = threadIdx.x;
@@ -242,9 +258,15 @@ __global__ void __launch_bounds__(kAdjBlock) k_adjoint(const AdjArgs A) {
   const char* tbl = A.tables + (size_t)cand * A.table_bytes;
   const SplHead* HD = reinterpret_cast<const SplHead*>(tbl);
   const SplInterval* IV = reinterpret_cast<const SplInterval*>(tbl + sizeof(SplHead) * A.pot.nspl);
-  double* gt = reinterpret_cast<double*>(dyn);                    // g coefficients, [record][NC]
+  const int ghb = GP ? A.pot.ng * (int)sizeof(SplHead) : 0;       // g heads (GP)
+  const SplHead* GH = reinterpret_cast<const SplHead*>(dyn);
+  double* gt = reinterpret_cast<double*>(dyn + ghb);              // g coefficients, [record][NC]
   double* atab = gt + (MEAM ? A.ngco * NC : 0);                   // coefficient + head adjoints
   double* gacc = atab + A.ntab;                                   // [g record][coef][replica]
+  if (GP) {
+    const int4* hsrc = reinterpret_cast<const int4*>(tbl + sizeof(SplHead) * A.pot.fn_off[4]);
+    for (int q = tid; q < ghb / 16; q += kAdjBlock) reinterpret_cast<int4*>(dyn)[q] = hsrc[q];
+  }
   if (MEAM) {
     const SplInterval* src = reinterpret_cast<const SplInterval*>(A.tables + (size_t)cand * A.table_bytes +
                                                                   sizeof(SplHead) * A.pot.nspl);
@@ -287,6 +309,7 @@ __global__ void __launch_bounds__(kAdjBlock) k_adjoint(const AdjArgs A) {
       push[0][p] = 0.0;
     }
     push[1][p] = sv;  // staged for S_i; cleared again below
+    if (GP) ptyp[p] = tj;
   }
   __syncthreads();
 
@@ -337,16 +360,8 @@ __global__ void __launch_bounds__(kAdjBlock) k_adjoint(const AdjArgs A) {
     const int kb = poff[ia], ke = poff[ia + 1];
     const int n = ke - kb, l = p - kb;
     const int trips = ((n - 1) >> 1) + (((n & 1) == 0 && l < (n >> 1)) ? 1 : 0);
-    const int sg = A.pot.fn_off[4] + ti;
-    const SplHead& Hg = HD[sg];
-    const double gx0 = Hg.x0, ginv = Hg.invstep, gxcut = Hg.xcut, gxn = Hg.xn;
-    const double gkmax = (double)(Hg.n - 2);
-    const double gy0 = Hg.y0, gyn = Hg.yn, ggL = Hg.gL, ggR = Hg.gR, gg2L = Hg.g2L, gg2R = Hg.g2R;
-    const bool gq = Hg.basis == 5;
-    const int grec = Hg.coef_off - A.gco0;
-    const double2* gtg = reinterpret_cast<const double2*>(gt + grec * NC);
-    double* gacl = gacc + grec * NC * R + (tid & (R - 1));
-    double* ghead = atab + ncoef * 6 + sg * 6;
+    const int gfix = GP ? ti * (nt * (nt + 1) / 2) : ti;  // g of the atom's type (and first pair with GP)
+    const GSpl G0 = gspl_of(GP ? GH[gfix] : HD[A.pot.fn_off[4] + ti], A.gco0);
     const double Aj = aj * dfj;
     const double Vjx = invr * (wx - aj * ux), Vjy = invr * (wy - aj * uy), Vjz = invr * (wz - aj * uz);
     double Rj = 0.0;
@@ -357,6 +372,10 @@ __global__ void __launch_bounds__(kAdjBlock) k_adjoint(const AdjArgs A) {
       const double4 Pa = make_double4(pd[0][k], pd[1][k], pd[2][k], pd[3][k]);
       const double4 Pb = make_double4(pd[4][k], pd[5][k], pd[6][k], pd[7][k]);
       const double cs = ux * Pa.x + uy * Pa.y + uz * Pa.z;
+      const int gi = GP ? gfix + pair_ij(tj, ptyp[k], nt) : gfix;
+      const GSpl G = GP ? gspl_of(GH[gi], A.gco0) : G0;
+      const double gx0 = G.x0, ginv = G.inv, gxcut = G.xcut, gxn = G.xn, gkmax = G.kmax;
+      const double2* gtg = reinterpret_cast<const double2*>(gt + G.rec * NC);
       double gv, dgt, t;
       int fl;
       bool extrap = false, keep = true;
@@ -364,8 +383,8 @@ __global__ void __launch_bounds__(kAdjBlock) k_adjoint(const AdjArgs A) {
         extrap = true;
         const bool left = cs < gx0;
         const double e = cs - (left ? gx0 : gxn);
-        const double g1 = left ? ggL : ggR, g2 = left ? gg2L : gg2R;
-        gv = (left ? gy0 : gyn) + g1 * e + 0.5 * g2 * e * e;
+        const double g1 = left ? G.gL : G.gR, g2 = left ? G.g2L : G.g2R;
+        gv = (left ? G.y0 : G.yn) + g1 * e + 0.5 * g2 * e * e;
         dgt = (g1 + g2 * e) / ginv;
         t = e;
         fl = left ? 0 : 1;
@@ -403,13 +422,14 @@ __global__ void __launch_bounds__(kAdjBlock) k_adjoint(const AdjArgs A) {
       if (POINT && extrap) {
         const bool left = fl == 0;
         const double e = t;
+        double* ghead = atab + ncoef * 6 + (A.pot.fn_off[4] + gi) * 6;
         atomicAdd(ghead + (left ? 0 : 1), al);
         atomicAdd(ghead + (left ? 2 : 3), al * e + be);
-        if (gq) atomicAdd(ghead + (left ? 4 : 5), 0.5 * al * e * e + be * e);
+        if (G.quintic) atomicAdd(ghead + (left ? 4 : 5), 0.5 * al * e * e + be * e);
       } else {
         const double bt = be * ginv;
         const double t2 = t * t, t3 = t2 * t;
-        double* cp = gacl + fl * (NC * R);
+        double* cp = gacc + G.rec * NC * R + (tid & (R - 1)) + fl * (NC * R);
         atomicAdd(cp, al);
         atomicAdd(cp + R, al * t + bt);
         atomicAdd(cp + 2 * R, al * t2 + 2.0 * bt * t);
@@ -528,7 +548,23 @@ __global__ void __launch_bounds__(64) k_chain(const ChainArgs A) {
     rb[n + 1] = hb[3];
     rb[n + 2] = A.basis[s] == 5 ? hb[4] : 0.0;
     rb[n + 3] = A.basis[s] == 5 ? hb[5] : 0.0;
-    if (A.meam && s >= A.f0 && s < A.f0 + A.nf) {
+    if (A.meam && A.fper && s >= A.f0 && s < A.f0 + A.nf) {
+      // per-spline f-max penalty (src/pots/pot_gmeam_spline/pot_gmeam_spline.cpp:275-280):
+      // the knot holding this spline's signed max |y| moves its term
+      const double* y = kv + A.koff[s];
+      double m = 0.0;
+      int km = -1;
+      for (int k = 0; k < n; ++k)
+        if (fabs(m) < fabs(y[k])) {
+          m = y[k];
+          km = k;
+        }
+      if (km >= 0) {
+        const double am = fabs(m);
+        const double pe = kDummyWeight * 25. * (1.0 - am) * (1.0 - am);
+        rb[km] += -2.0 * pe * kDummyWeight * 25. * 2.0 * (1.0 - am) * (m > 0 ? 1.0 : -1.0);
+      }
+    } else if (A.meam && s >= A.f0 && s < A.f0 + A.nf) {
       // f-max penalty (src/pots/pot_meam_spline/pot_meam_spline.cpp:274-278):
       // only the knot holding the signed max |y| over all f splines moves it
       double mx = 0.0;
@@ -739,7 +775,8 @@ int setup_adjoint(mz_ctx* c) {
   // limit: replicas (fewer lanes adding into one address) were measured to
   // matter more than workgroups per CU.
   const int nc_g = c->pot.quintic ? 6 : 4;
-  const int fixed = (X.ngco * nc_g + X.ntab) * (int)sizeof(double);
+  const int fixed = (X.ngco * nc_g + X.ntab) * (int)sizeof(double) +
+                    (c->pot.gpair ? c->pot.ng * (int)sizeof(mz::SplHead) : 0);
   const int stat = (int)(mz::kPartnerFields * sizeof(double) * mz::kAdjBlock + 2 * sizeof(double) * mz::kAdjBlock +
                          2 * sizeof(double) * mz::kAdjAtoms + sizeof(int) * (mz::kAdjAtoms + 1));
   X.grep_log2 = -1;
@@ -777,13 +814,19 @@ int ensure_adjoint_buffers(mz_ctx* c, int ncand) {
   return MZ_OK;
 }
 
+template <bool M, bool P, bool Q, bool GP>
+void launch_adjoint_g(hipStream_t st, const mz::AdjArgs& a, int nchunks, int ncand, int dyn) {
+  const dim3 grid(nchunks, ncand), block(mz::kAdjBlock);
+  if (a.grep_log2 == 5) hipLaunchKernelGGL((mz::k_adjoint<M, P, Q, 32, GP>), grid, block, dyn, st, a);
+  else if (a.grep_log2 == 3) hipLaunchKernelGGL((mz::k_adjoint<M, P, Q, 8, GP>), grid, block, dyn, st, a);
+  else hipLaunchKernelGGL((mz::k_adjoint<M, P, Q, 1, GP>), grid, block, dyn, st, a);
+}
+
 template <bool M, bool P, bool Q>
 void launch_adjoint_t(hipStream_t st, const mz::AdjArgs& a, int nchunks, int ncand, int dyn) {
-  const dim3 grid(nchunks, ncand), block(mz::kAdjBlock);
-  if (!M) hipLaunchKernelGGL((mz::k_adjoint<M, P, Q, 1>), grid, block, dyn, st, a);
-  else if (a.grep_log2 == 5) hipLaunchKernelGGL((mz::k_adjoint<M, P, Q, 32>), grid, block, dyn, st, a);
-  else if (a.grep_log2 == 3) hipLaunchKernelGGL((mz::k_adjoint<M, P, Q, 8>), grid, block, dyn, st, a);
-  else hipLaunchKernelGGL((mz::k_adjoint<M, P, Q, 1>), grid, block, dyn, st, a);
+  if (!M) hipLaunchKernelGGL((mz::k_adjoint<M, P, Q, 1, false>), dim3(nchunks, ncand), dim3(mz::kAdjBlock), dyn, st, a);
+  else if (a.pot.gpair) launch_adjoint_g<M, P, Q, true>(st, a, nchunks, ncand, dyn);
+  else launch_adjoint_g<M, P, Q, false>(st, a, nchunks, ncand, dyn);
 }
 
 }  // namespace
@@ -853,6 +896,7 @@ static int run_grad(mz_ctx* c, int ncand) {
   ca.nt = c->nt;
   ca.f0 = c->pot.fn_off[3];
   ca.nf = c->nt * (c->nt + 1) / 2;
+  ca.fper = c->meam && c->nfpen == ca.nf && (c->style == mz::kGMeamSpline || c->style == mz::kGMeam);
   ca.u0 = c->pot.fn_off[2];
   ca.nknots = c->d_nknots.p;
   ca.basis = c->d_basis.p;

@@ -82,6 +82,7 @@ struct mz_ctx {
   mz::PotDesc pot{};
   int style = 0, nt = 0, nspl = 0;
   bool meam = false, point = false;
+  int nfpen = 0;  // f-max penalty terms: 1 (meam), one per f spline (gmeam)
   std::vector<int> basis, nknots, bc, koff, coef_off;
   std::vector<double> x0, xn;
   std::vector<mz::Grid> grids;

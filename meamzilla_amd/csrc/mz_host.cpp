@@ -45,14 +45,16 @@ int pair_ij_h(int ti, int tj, int nt) {
 namespace {
 
 int build_layout(mz_ctx* c, const mz_layout* L) {
-  if (!L || L->style < 0 || L->style > 3) return fail(MZ_ERR_LAYOUT, "unknown style");
+  if (!L || L->style < 0 || L->style > 5) return fail(MZ_ERR_LAYOUT, "unknown style");
   if (L->ntypes < 1 || L->ntypes > mz::kMaxTypes) return fail(MZ_ERR_LAYOUT, "ntypes out of range");
   c->style = L->style;
   c->nt = L->ntypes;
-  c->meam = (L->style == mz::kMeamSpline || L->style == mz::kMeam);
-  c->point = (L->style == mz::kEam || L->style == mz::kMeam);
+  const bool gmeam = L->style == mz::kGMeamSpline || L->style == mz::kGMeam;
+  c->meam = (L->style == mz::kMeamSpline || L->style == mz::kMeam || gmeam);
+  c->point = (L->style == mz::kEam || L->style == mz::kMeam || L->style == mz::kGMeam);
   const int nt = c->nt, npij = nt * (nt + 1) / 2;
-  const int counts[5] = {npij, nt, nt, npij, nt};
+  // g: Triplet_i (meam), Triplet_i_jk (gmeam, src/pots/pot_gmeam/pot_gmeam.cpp:31-36)
+  const int counts[5] = {npij, nt, nt, npij, gmeam ? nt * npij : nt};
   const int nfn = c->meam ? 5 : 3;
   int tot = 0;
   for (int f = 0; f < 5; ++f) c->pot.fn_off[f] = 0;
@@ -101,12 +103,15 @@ int build_layout(mz_ctx* c, const mz_layout* L) {
   c->pot.quintic = quintic;
   c->pot.gco0 = c->meam ? c->coef_off[c->pot.fn_off[4]] : coff;
   c->pot.ngco = c->meam ? coff - c->pot.gco0 : 0;
+  c->pot.ng = c->meam ? counts[4] : 0;
+  c->pot.gpair = gmeam && nt > 1;  // one type: gmeam is meam
+  c->nfpen = gmeam ? npij : (c->meam ? 1 : 0);
   // |cos| <= 1 + a few ulps, so a knot-style g spline ending at or beyond 1
   // never cuts a triplet
   c->pot.g_keep_all = 0;
   if (c->meam && !c->point) {
     bool all = true;
-    for (int t = 0; t < nt; ++t) {
+    for (int t = 0; t < counts[4]; ++t) {
       const int s = c->pot.fn_off[4] + t;
       if (!(c->xn[s] * (1. + 1.e-8) > 1.0 + 1.e-12)) all = false;
     }
@@ -428,6 +433,7 @@ mz::FinalArgs final_args(const mz_ctx* c) {
   f.nknot = c->nknot;
   f.f0 = c->pot.fn_off[3];
   f.nf = c->nt * (c->nt + 1) / 2;
+  f.fper = c->meam && c->nfpen == f.nf && (c->style == mz::kGMeamSpline || c->style == mz::kGMeam);
   f.u0 = c->pot.fn_off[2];
   f.koff = c->d_koff.p;
   f.nknots = c->d_nknots.p;
@@ -458,7 +464,7 @@ int mz_run_forward(mz_ctx* c, int ncand, double* d_errvec, double* d_ce, double*
   // candidates per workgroup: the tile's choice, at most what ncand fills and
   // what the LDS holds
   int ct = std::min(c->tile.cand_tile, ncand >= 4 ? 4 : (ncand >= 2 ? 2 : 1));
-  while (ct > 1 && mz::atoms_static_lds(c->tile.block, ppl, ct) + mz::atoms_dyn_lds(c->pot, ct) > 64 * 1024) ct >>= 1;
+  while (ct > 1 && mz::atoms_static_lds(c->tile.block, ppl, ct, c->pot.gpair != 0) + mz::atoms_dyn_lds(c->pot, ct) > 64 * 1024) ct >>= 1;
   if (mz::launch_atoms(st, aa, (int)c->chunks.size(), ncand, c->tile.block, ppl, ct))
     return fail(MZ_ERR_ARG, "unsupported tile (block, pairs per lane)");
   MZ_HIP(hipGetLastError());
@@ -569,7 +575,7 @@ int mz_create(const mz_layout* layout, const mz_db* db, mz_ctx** out) {
     na += c->natoms_c[k];
   }
   c->natoms = na;
-  c->nerr = 7 * c->ncells + 3 * na + c->ncells + (c->meam ? 1 : c->nt);
+  c->nerr = 7 * c->ncells + 3 * na + c->ncells + (c->meam ? c->nfpen : c->nt);
   c->types.assign(db->types, db->types + na);
   c->cell_of.resize(na);
   for (int k = 0; k < c->ncells; ++k)
@@ -799,7 +805,7 @@ int mz_set_tile(mz_ctx* c, const mz_tile* t) {
   if (!ok || ppl * t->block != t->max_pairs) return fail(MZ_ERR_ARG, "unsupported tile (block, max_pairs)");
   if (t->cand_tile != 0 && t->cand_tile != 1 && t->cand_tile != 2 && t->cand_tile != 4)
     return fail(MZ_ERR_ARG, "cand_tile must be 1, 2 or 4");
-  if (mz::atoms_static_lds(t->block, ppl, 1) + mz::atoms_dyn_lds(c->pot, 1) > 64 * 1024)
+  if (mz::atoms_static_lds(t->block, ppl, 1, c->pot.gpair != 0) + mz::atoms_dyn_lds(c->pot, 1) > 64 * 1024)
     return fail(MZ_ERR_CAPACITY, "tile and spline tables exceed 64 KB of LDS");
   mz_tile old = c->tile;
   drop_graphs(c);

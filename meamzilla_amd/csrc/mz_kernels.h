@@ -81,6 +81,7 @@ struct FinalArgs {
   int meam, quintic;
   int ncells, natoms, nerr, nt, nspl, nknot;
   int f0, nf, u0;
+  int fper;               // one f-max penalty per f spline (gmeam) instead of one over all
   const int* koff;
   const int* nknots;
   const double* knots;
@@ -98,7 +99,7 @@ int launch_atoms(hipStream_t st, const AtomArgs& a, int nchunks, int ncand, int 
 // Dynamic (spline tables) and static LDS bytes of one atom workgroup.
 bool atoms_tables_global(const PotDesc& pot, int ct);  // k_atoms reads spline tables from global memory
 size_t atoms_dyn_lds(const PotDesc& pot, int ct);
-size_t atoms_static_lds(int block, int ppl, int ct);
+size_t atoms_static_lds(int block, int ppl, int ct, bool gpair);
 int resid_blocks(int natoms, int ncells);
 void launch_resid(hipStream_t st, const ResidArgs& a, int nblocks, int ncand);
 void launch_final(hipStream_t st, const FinalArgs& a, int ncand);

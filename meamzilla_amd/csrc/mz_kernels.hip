@@ -314,7 +314,7 @@ __device__ __forceinline__ void g_value(const SplHead& Hg, const double* gtg, co
 
 // LDS layout of one workgroup (static part).  CT candidates share the pair
 // geometry; f of candidate 0 rides in the partner record, the others in fx.
-template <int BLOCK, int PPL, int MAXA, int CT>
+template <int BLOCK, int PPL, int MAXA, int CT, bool GP>
 struct AtomSmem {
   double4 rec[BLOCK * PPL];                 // per pair: unit vector, f (candidate 0); later T
   double fx[CT > 1 ? CT - 1 : 1][BLOCK * PPL];  // f of candidates 1..CT-1
@@ -323,14 +323,17 @@ struct AtomSmem {
   double dF[CT][MAXA];
   int poff[MAXA + 1];
   double red8[CT][8][BLOCK / 64];
+  int ptype[GP ? BLOCK * PPL : 1];          // per pair: type of its neighbour (g by pair types)
 };
 
 // One workgroup per (chunk, group of CT candidates).  TG: the spline tables
 // are read from global memory instead of being staged in LDS (large tables).
-template <int BLOCK, int PPL, int MAXA, int CT, bool MEAM, bool POINT, bool Q, bool TG>
+// GP: the g spline of a triplet depends on the types of both neighbours
+// (gmeam styles, src/pot_fns.cpp:154-171 Triplet_i_jk), chosen per triplet.
+template <int BLOCK, int PPL, int MAXA, int CT, bool MEAM, bool POINT, bool Q, bool TG, bool GP>
 __global__ void __launch_bounds__(BLOCK) k_atoms(const AtomArgs args) {
   extern __shared__ __attribute__((aligned(64))) char dyn[];
-  __shared__ __attribute__((aligned(16))) AtomSmem<BLOCK, PPL, MAXA, CT> sm;
+  __shared__ __attribute__((aligned(16))) AtomSmem<BLOCK, PPL, MAXA, CT, GP> sm;
   constexpr int NG = g_record_doubles(Q);
   const Chunk ch = args.chunks[blockIdx.x];
   const int tid = threadIdx.x;
@@ -350,7 +353,7 @@ __global__ void __launch_bounds__(BLOCK) k_atoms(const AtomArgs args) {
   // intervals again as compact power-of-two records (c0..c3, or c0..c5 padded
   // to 8 doubles), which the triplet walk reads per triplet.
   const int gbytes = MEAM ? args.pot.ngco * NG * 8 : 0;
-  const int ghb = MEAM && TG ? args.pot.nt * (int)sizeof(SplHead) : 0;
+  const int ghb = MEAM && TG ? args.pot.ng * (int)sizeof(SplHead) : 0;
   const int tbytes = TG ? 0 : TB;
   const int cbytes = tbytes + ghb + gbytes;
 #pragma unroll
@@ -376,7 +379,7 @@ __global__ void __launch_bounds__(BLOCK) k_atoms(const AtomArgs args) {
     return reinterpret_cast<const SplHead*>(TG ? args.tables + (size_t)cand[c] * TB : dyn + (size_t)c * cbytes);
   };
   auto IVc = [&](int c) { return reinterpret_cast<const SplInterval*>(HDc(c) + args.pot.nspl); };
-  auto GHc = [&](int c) {  // g heads by element type
+  auto GHc = [&](int c) {  // g heads by element type (by type and pair of types with GP)
     return TG ? reinterpret_cast<const SplHead*>(dyn + (size_t)c * cbytes) : HDc(c) + args.pot.fn_off[4];
   };
   auto GTc = [&](int c) { return reinterpret_cast<const double*>(dyn + (size_t)c * cbytes + tbytes + ghb); };
@@ -389,7 +392,7 @@ __global__ void __launch_bounds__(BLOCK) k_atoms(const AtomArgs args) {
   const SplInterval* IV0 = IVc(0);
 
   // per-lane pair state
-  int ia[PPL];
+  int ia[PPL], tjs[PPL];
   double r[PPL], invr[PPL], ux[PPL], uy[PPL], uz[PPL];
   double aphi[PPL][CT], brho[PPL][CT], fv[PPL][CT], dfv[PPL][CT], Gx[PPL][CT], Gy[PPL][CT], Gz[PPL][CT],
       S1v[PPL][CT];
@@ -438,6 +441,8 @@ __global__ void __launch_bounds__(BLOCK) k_atoms(const AtomArgs args) {
       uy[s] = d[1] / rr;
       uz[s] = d[2] / rr;
       const int ti = args.types[ig], tj = args.types[j];
+      tjs[s] = tj;
+      if (GP) sm.ptype[p] = tj;
       const int pij = pair_ij(ti, tj, nt);
       int k, reg;
       double t;
@@ -490,8 +495,9 @@ __global__ void __launch_bounds__(BLOCK) k_atoms(const AtomArgs args) {
       if (ia[s] < 0) continue;
       const int kb = sm.poff[ia[s]], ke = sm.poff[ia[s] + 1];
       const int ti = args.types[ch.a0 + ia[s]];
-      const SplHead& Hg = GHc(0)[ti];
-      const int goff = (Hg.coef_off - args.pot.gco0) * NG;
+      const int gfix = GP ? ti * (nt * (nt + 1) / 2) : ti;  // g of the lane's atom (and, with GP, first pair)
+      const SplHead& Hg0 = GHc(0)[gfix];
+      const int goff0 = (Hg0.coef_off - args.pot.gco0) * NG;
       const double xj = ux[s], yj = uy[s], zj = uz[s];
       double S[CT][5];
 #pragma unroll
@@ -501,12 +507,15 @@ __global__ void __launch_bounds__(BLOCK) k_atoms(const AtomArgs args) {
       for (int k = kb; k < ke; ++k) {
         const double4 P = sm.rec[k];
         const double cs = xj * P.x + yj * P.y + zj * P.z;
+        const int gi = GP ? gfix + pair_ij(tjs[s], sm.ptype[k], nt) : gfix;
+        const SplHead& Hg = GP ? GHc(0)[gi] : Hg0;
+        const int goff = GP ? (Hg.coef_off - args.pot.gco0) * NG : goff0;
         const GLoc L = g_locate<POINT, Q>(Hg, cs);
         const bool zero = k == p || (cut && !(cs < Hg.xcut));
 #pragma unroll
         for (int c = 0; c < CT; ++c) {
           double gv, gp;
-          g_value<POINT, Q>(GHc(c)[ti], GTc(c) + goff, L, gv, gp);
+          g_value<POINT, Q>(GHc(c)[gi], GTc(c) + goff, L, gv, gp);
           const double fk = zero ? 0.0 : (c == 0 ? P.w : sm.fx[c > 0 ? c - 1 : 0][k]);
           const double b = gp * fk;
           S[c][0] += gv * fk;
@@ -732,7 +741,19 @@ __global__ void __launch_bounds__(256) k_final(const FinalArgs a) {
   double* ev = a.errvec ? a.errvec + (size_t)cand * a.nerr : nullptr;
   const size_t tail = 8 * (size_t)a.ncells + 3 * (size_t)a.natoms;
   const double* kv = a.knots + (size_t)cand * a.nknot;
-  if (a.meam) {
+  if (a.meam && a.fper) {
+    // one term per f spline (src/pots/pot_gmeam_spline/pot_gmeam_spline.cpp:275-280)
+    for (int q = 0; q < a.nf; ++q) {
+      const int s2 = a.f0 + q;
+      const double* y = kv + a.koff[s2];
+      double m = 0.0;
+      for (int k = 0; k < a.nknots[s2]; ++k)
+        if (fabs(m) < fabs(y[k])) m = y[k];
+      const double pe = kDummyWeight * 25. * (1.0 - fabs(m)) * (1.0 - fabs(m));
+      F += pe * pe;
+      if (ev) ev[tail + q] = pe;
+    }
+  } else if (a.meam) {
     // PotFns::get_max_y_mag over the f splines (src/pot_fns.cpp:100-111)
     double mx = 0.0;
     for (int q = 0; q < a.nf; ++q) {
@@ -781,23 +802,26 @@ void launch_zero(hipStream_t st, double* p, size_t n) {
 template <int BLOCK, int PPL, int MAXA, int CT>
 static void launch_atoms_t(hipStream_t st, const AtomArgs& a, int nchunks, int ncand) {
   const int style = a.pot.style;
-  const bool meam = style == kMeamSpline || style == kMeam;
-  const bool point = style == kEam || style == kMeam;
+  const bool meam = style == kMeamSpline || style == kMeam || style == kGMeamSpline || style == kGMeam;
+  const bool point = style == kEam || style == kMeam || style == kGMeam;
   const bool q = a.pot.quintic != 0;
   const size_t dyn = atoms_dyn_lds(a.pot, CT);
-  const bool tg = atoms_tables_global(a.pot, CT);
+  const bool tg = atoms_tables_global(a.pot, CT);  // always with gpair
   dim3 grid(nchunks, (ncand + CT - 1) / CT), block(BLOCK);
-#define MZ_LAUNCH(M, P, Q_, TG_)                                                                    \
-  hipLaunchKernelGGL((k_atoms<BLOCK, PPL, MAXA, CT, M, P, Q_, TG_>), grid, block, dyn, st, a)
-  if (meam && tg) {
-    if (point) { if (q) MZ_LAUNCH(true, true, true, true); else MZ_LAUNCH(true, true, false, true); }
-    else { if (q) MZ_LAUNCH(true, false, true, true); else MZ_LAUNCH(true, false, false, true); }
+#define MZ_LAUNCH(M, P, Q_, TG_, GP_)                                                              \
+  hipLaunchKernelGGL((k_atoms<BLOCK, PPL, MAXA, CT, M, P, Q_, TG_, GP_>), grid, block, dyn, st, a)
+  if (meam && a.pot.gpair) {
+    if (point) { if (q) MZ_LAUNCH(true, true, true, true, true); else MZ_LAUNCH(true, true, false, true, true); }
+    else { if (q) MZ_LAUNCH(true, false, true, true, true); else MZ_LAUNCH(true, false, false, true, true); }
+  } else if (meam && tg) {
+    if (point) { if (q) MZ_LAUNCH(true, true, true, true, false); else MZ_LAUNCH(true, true, false, true, false); }
+    else { if (q) MZ_LAUNCH(true, false, true, true, false); else MZ_LAUNCH(true, false, false, true, false); }
   } else if (meam) {
-    if (point) { if (q) MZ_LAUNCH(true, true, true, false); else MZ_LAUNCH(true, true, false, false); }
-    else { if (q) MZ_LAUNCH(true, false, true, false); else MZ_LAUNCH(true, false, false, false); }
+    if (point) { if (q) MZ_LAUNCH(true, true, true, false, false); else MZ_LAUNCH(true, true, false, false, false); }
+    else { if (q) MZ_LAUNCH(true, false, true, false, false); else MZ_LAUNCH(true, false, false, false, false); }
   } else {
-    if (point) { if (q) MZ_LAUNCH(false, true, true, false); else MZ_LAUNCH(false, true, false, false); }
-    else { if (q) MZ_LAUNCH(false, false, true, false); else MZ_LAUNCH(false, false, false, false); }
+    if (point) { if (q) MZ_LAUNCH(false, true, true, false, false); else MZ_LAUNCH(false, true, false, false, false); }
+    else { if (q) MZ_LAUNCH(false, false, true, false, false); else MZ_LAUNCH(false, false, false, false, false); }
   }
 #undef MZ_LAUNCH
 }
@@ -807,21 +831,21 @@ static void launch_atoms_t(hipStream_t st, const AtomArgs& a, int nchunks, int n
 // potential's 17.5 KB tables cost a candidate per workgroup and a workgroup
 // per CU; the one-element potential's 3.9 KB are faster staged).
 bool atoms_tables_global(const PotDesc& pot, int ct) {
-  const bool meam = pot.style == kMeamSpline || pot.style == kMeam;
-  return meam && (size_t)ct * (size_t)pot.table_bytes > 16 * 1024;
+  const bool meam = pot.ngco > 0;
+  return meam && (pot.gpair || (size_t)ct * (size_t)pot.table_bytes > 16 * 1024);
 }
 
 size_t atoms_dyn_lds(const PotDesc& pot, int ct) {
-  const bool meam = pot.style == kMeamSpline || pot.style == kMeam;
+  const bool meam = pot.ngco > 0;
   const size_t g = meam ? (size_t)pot.ngco * 8 * g_record_doubles(pot.quintic != 0) : 0;
-  if (atoms_tables_global(pot, ct)) return (size_t)ct * ((size_t)pot.nt * sizeof(SplHead) + g);
+  if (atoms_tables_global(pot, ct)) return (size_t)ct * ((size_t)pot.ng * sizeof(SplHead) + g);
   return (size_t)ct * ((size_t)pot.table_bytes + g);
 }
 
-size_t atoms_static_lds(int block, int ppl, int ct) {
+size_t atoms_static_lds(int block, int ppl, int ct, bool gpair) {
   const size_t P = (size_t)block * ppl;
   return P * 32 + (size_t)(ct > 1 ? ct - 1 : 1) * P * 8 + (size_t)ct * P * 8 + (size_t)ct * 64 * 8 * 2 + 65 * 4 +
-         (size_t)ct * 8 * (block / 64) * 8;
+         (size_t)ct * 8 * (block / 64) * 8 + (gpair ? P * 4 : 4);
 }
 
 int launch_atoms(hipStream_t st, const AtomArgs& a, int nchunks, int ncand, int block, int ppl, int ct) {

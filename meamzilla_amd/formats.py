@@ -19,19 +19,30 @@ from typing import List, Optional, Sequence
 
 import numpy as np
 
-STYLES = ("eam/spline", "meam/spline", "eam", "meam")
+STYLES = ("eam/spline", "meam/spline", "eam", "meam", "gmeam/spline", "gmeam")
 
 
 def n_pair_ij(nt: int) -> int:
     return nt * (nt + 1) // 2
 
 
+def is_meam_style(style: str) -> bool:
+    """Styles with the f and g (three-body) functions."""
+    return style.startswith(("meam", "gmeam"))
+
+
+def is_gmeam_style(style: str) -> bool:
+    """Styles whose g splines are indexed by (type i, types j k) (Triplet_i_jk)."""
+    return style.startswith("gmeam")
+
+
 def function_counts(style: str, nt: int) -> List[int]:
     """Alloy members per function in file order (src/pot_fns.cpp:280-310):
-    phi Pair_ij, rho Pair_j, U Pair_i, f Pair_ij, g Triplet_i."""
+    phi Pair_ij, rho Pair_j, U Pair_i, f Pair_ij, g Triplet_i (gmeam styles:
+    Triplet_i_jk, nt * nt (nt + 1) / 2 members, src/pots/pot_gmeam/pot_gmeam.cpp:31-36)."""
     counts = [n_pair_ij(nt), nt, nt]
-    if style.startswith("meam"):
-        counts += [n_pair_ij(nt), nt]
+    if is_meam_style(style):
+        counts += [n_pair_ij(nt), nt * n_pair_ij(nt) if is_gmeam_style(style) else nt]
     return counts
 
 
@@ -103,7 +114,7 @@ class PotentialSpec:
 
     def max_rcut(self) -> float:
         counts = function_counts(self.style, self.ntypes)
-        radial = [0, 1] + ([3] if self.style.startswith("meam") else [])
+        radial = [0, 1] + ([3] if is_meam_style(self.style) else [])
         r = 0.0
         off = 0
         for fi, c in enumerate(counts):

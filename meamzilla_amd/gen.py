@@ -30,7 +30,7 @@ from typing import Callable, List, Optional, Tuple
 
 import numpy as np
 
-from .formats import Database, PotentialSpec, Spline, function_counts
+from .formats import Database, PotentialSpec, Spline, function_counts, is_meam_style
 
 GPA_TO_EV_A3 = 1.0 / 160.21766208
 NOISE_E = 1e-3          # eV/atom
@@ -178,7 +178,7 @@ def make_potential(rng, style: str, nt: int, rc: float, nk: dict, u_range, basis
     spl += [_radial(rng, 1.8, rc, nk["phi"], -0.2, 0.4, basis) for _ in range(counts[0])]
     spl += [_radial(rng, 1.8, rc, nk["rho"], 0.0, rho_amp, basis) for _ in range(counts[1])]
     spl += [_flat(rng, u_range[0], u_range[1], nk["U"], -0.5, 0.5, basis) for _ in range(counts[2])]
-    if style.startswith("meam"):
+    if is_meam_style(style):
         fs = [_radial(rng, 1.8, rc, nk["f"], -1.0, 1.0, basis) for _ in range(counts[3])]
         m = max(np.abs(s.y).max() for s in fs)
         for s in fs:

@@ -35,6 +35,8 @@ int style_code(const std::string& s) {
   if (s == "meam/spline") return MZ_STYLE_MEAM_SPLINE;
   if (s == "eam") return MZ_STYLE_EAM;
   if (s == "meam") return MZ_STYLE_MEAM;
+  if (s == "gmeam/spline") return MZ_STYLE_GMEAM_SPLINE;
+  if (s == "gmeam") return MZ_STYLE_GMEAM;
   throw Error("invalid potential style \"" + s + "\"");
 }
 
@@ -57,6 +59,7 @@ int splines_for_style(const std::string& style, int nt) {
   const int npij = nt * (nt + 1) / 2;
   const int eam = npij + nt + nt;
   style_code(style);
+  if (style.compare(0, 5, "gmeam") == 0) return eam + npij + nt * npij;  // g Triplet_i_jk
   return (style.compare(0, 4, "meam") == 0) ? eam + npij + nt : eam;
 }
 
@@ -1061,7 +1064,8 @@ void write_data_files(const std::string& base, Potential& pot, const ConfigDb& d
   const int nc = db.ncells;
   int natoms = 0;
   for (int c = 0; c < nc; ++c) natoms += db.natoms[c];
-  const bool meam = pot.style().compare(0, 4, "meam") == 0;
+  const bool gmeam = pot.style().compare(0, 5, "gmeam") == 0;
+  const bool meam = gmeam || pot.style().compare(0, 4, "meam") == 0;
   using std::setw;
   using std::setprecision;
   {  // densities (src/pots/pot_eam/pot_eam.cpp:506-543)
@@ -1157,7 +1161,18 @@ void write_data_files(const std::string& base, Potential& pot, const ConfigDb& d
       o << setw(3) << c << std::fixed << setw(11) << setprecision(4) << db.weight[c] << setw(15) << setprecision(6)
         << ev[idx] * ev[idx] << setw(15) << ev[idx] / db.weight[c] << "\n";
     const int nt = pot.ntypes();
-    if (meam) {
+    if (gmeam) {
+      // one line per f spline (src/pots/pot_gmeam/pot_gmeam.cpp:246-292)
+      const int f0 = nt * (nt + 1) / 2 + 2 * nt, nf = nt * (nt + 1) / 2;
+      o << "\nDummy constraints (|f_max(r)| = 1)\n";
+      o << setw(15) << "punish error" << setw(15) << "|f_max(r)|" << "\n";
+      for (int q = f0; q < f0 + nf; ++q, ++idx) {
+        double m = 0.0;
+        for (double y : pot.splines()[q].y)
+          if (std::abs(m) < std::abs(y)) m = y;
+        o << std::fixed << setw(15) << setprecision(6) << ev[idx] * ev[idx] << setw(15) << std::abs(m) << "\n";
+      }
+    } else if (meam) {
       // signed knot value of largest magnitude over the f splines (src/pot_fns.cpp:100-111)
       const int f0 = nt * (nt + 1) / 2 + 2 * nt, nf = nt * (nt + 1) / 2;
       double fmax = 0.0;

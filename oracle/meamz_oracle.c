@@ -321,12 +321,24 @@ double orc_splint_point(int basis, int n, double x0, double xn, const int* bc, c
 
 /* ---------------------------------------------------------------- context */
 
-static int is_meam(const orc_ctx* c) { return c->style == ORC_MEAM_SPLINE || c->style == ORC_MEAM; }
-static int is_spline_style(const orc_ctx* c) { return c->style == ORC_EAM_SPLINE || c->style == ORC_MEAM_SPLINE; }
+static int is_gmeam(const orc_ctx* c) { return c->style == ORC_GMEAM_SPLINE || c->style == ORC_GMEAM; }
+static int is_meam(const orc_ctx* c) { return c->style == ORC_MEAM_SPLINE || c->style == ORC_MEAM || is_gmeam(c); }
+static int is_spline_style(const orc_ctx* c) {
+  return c->style == ORC_EAM_SPLINE || c->style == ORC_MEAM_SPLINE || c->style == ORC_GMEAM_SPLINE;
+}
 
 /* PotFns::get_2body_alloy_idx Pair_ij, src/pot_fns.cpp:141-143 */
 static int pair_ij(int ti, int tj, int nt) {
   return (ti <= tj) ? ti * nt + tj - ((ti * (ti + 1)) / 2) : tj * nt + ti - ((tj * (tj + 1)) / 2);
+}
+
+/* g spline of a triplet: PotFns::get_3body_alloy_idx src/pot_fns.cpp:154-171,
+ * Triplet_i for meam styles (src/pots/pot_meam/pot_meam.cpp:45-47), Triplet_i_jk
+ * for gmeam styles (src/pots/pot_gmeam/pot_gmeam.cpp:31-36). */
+static const spl_t* g_spline(const orc_ctx* c, int ti, int tj, int tk) {
+  const int nt = c->nt;
+  const int idx = is_gmeam(c) ? ti * (nt * (nt + 1) / 2) + pair_ij(tj, tk, nt) : ti;
+  return &c->spl[c->fn_off[4] + idx];
 }
 
 static void refresh_all(orc_ctx* c, const double* knots) {
@@ -354,7 +366,7 @@ orc_ctx* orc_create(const orc_layout* lay, const orc_db* db, char* err) {
   c->nspl = lay->nspl;
   c->embed_extrap = lay->embed_extrap;
   int nt = c->nt, npij = nt * (nt + 1) / 2;
-  int counts[5] = {npij, nt, nt, npij, nt};
+  int counts[5] = {npij, nt, nt, npij, is_gmeam(c) ? nt * npij : nt};
   int nfn = is_meam(c) ? 5 : 3;
   int tot = 0;
   for (int f = 0; f < nfn; ++f) { c->fn_off[f] = tot; tot += counts[f]; }
@@ -520,12 +532,12 @@ orc_ctx* orc_create(const orc_layout* lay, const orc_db* db, char* err) {
     if (is_meam(c)) {
       long long b = c->poff[a], e = c->poff[a + 1];
       int ti = c->types[a];
-      const spl_t* g = &c->spl[c->fn_off[4] + ti];
       for (long long p = b; p < e - 1; ++p)
         for (long long q = p + 1; q < e; ++q) {
           const pair_t* P = &c->pairs[p];
           const pair_t* Q = &c->pairs[q];
-          if (c->style == ORC_MEAM_SPLINE) {
+          const spl_t* g = g_spline(c, ti, c->types[P->j], c->types[Q->j]);
+          if (is_spline_style(c)) {
             double cs = P->dist[0] * Q->dist[0] + P->dist[1] * Q->dist[1] + P->dist[2] * Q->dist[2];
             double sh;
             if (spl_knotshift(g, cs, &sh) != -1) ++nt_a;
@@ -557,7 +569,7 @@ void orc_counts(const orc_ctx* c, long long* npairs, long long* ntriplets) {
 }
 
 int orc_errvec_size(const orc_ctx* c) {
-  return 7 * c->ncells + 3 * c->natoms + c->ncells + (is_meam(c) ? 1 : c->nt);
+  return 7 * c->ncells + 3 * c->natoms + c->ncells + (is_gmeam(c) ? c->nt * (c->nt + 1) / 2 : is_meam(c) ? 1 : c->nt);
 }
 
 /* ------------------------------------------------------------ objective core */
@@ -613,15 +625,16 @@ static void atom_body(const orc_ctx* c, int a, accum_t* acc) {
       }
     }
   }
-  const spl_t* g = meam ? &c->spl[c->fn_off[4] + c->types[a]] : NULL;
+  const int ti = c->types[a];
   if (meam) {
     for (long long p = b; p < e - 1; ++p)
       for (long long q = p + 1; q < e; ++q) {
         const pair_t* P = &c->pairs[p];
         const pair_t* Q = &c->pairs[q];
         double cs = P->dist[0] * Q->dist[0] + P->dist[1] * Q->dist[1] + P->dist[2] * Q->dist[2];
+        const spl_t* g = g_spline(c, ti, c->types[P->j], c->types[Q->j]);
         double gv, dg;
-        if (c->style == ORC_MEAM_SPLINE) {
+        if (is_spline_style(c)) {
           double sh;
           int kn = spl_knotshift(g, cs, &sh);
           if (kn == -1) continue;
@@ -662,8 +675,9 @@ static void atom_body(const orc_ctx* c, int a, accum_t* acc) {
         const pair_t* P = &c->pairs[p];
         const pair_t* Q = &c->pairs[q];
         double cs = P->dist[0] * Q->dist[0] + P->dist[1] * Q->dist[1] + P->dist[2] * Q->dist[2];
+        const spl_t* g = g_spline(c, ti, c->types[P->j], c->types[Q->j]);
         double gv, dg;
-        if (c->style == ORC_MEAM_SPLINE) {
+        if (is_spline_style(c)) {
           double sh;
           int kn = spl_knotshift(g, cs, &sh);
           if (kn == -1) continue;
@@ -742,7 +756,17 @@ static double finish(const orc_ctx* c, const accum_t* acc, double* errvec, doubl
     if (errvec) errvec[cnt] = v;
     ++cnt;
   }
-  if (is_meam(c)) {
+  if (is_gmeam(c)) {
+    /* one f-max term per f spline, src/pots/pot_gmeam_spline/pot_gmeam_spline.cpp:275-280 */
+    int npij = c->nt * (c->nt + 1) / 2;
+    for (int s = c->fn_off[3]; s < c->fn_off[3] + npij; ++s) {
+      double m = fabs(spl_max_y_mag(&c->spl[s]));
+      double pe = DUMMY_WEIGHT * 25. * (1.0 - m) * (1.0 - m);
+      F += pe * pe;
+      if (errvec) errvec[cnt] = pe;
+      ++cnt;
+    }
+  } else if (is_meam(c)) {
     /* PotFns::get_max_y_mag src/pot_fns.cpp:100-111 over the f splines */
     int npij = c->nt * (c->nt + 1) / 2;
     double mx = 0.0;
@@ -880,14 +904,14 @@ void orc_densities(orc_ctx* c, const double* knots, double* dens) {
       if (meam) f[p] = (P->f_knot != -1) ? eval_fn(c, &c->spl[P->f_idx], P->f_knot, P->f_shift, P->r, &tmp) : 0.0;
     }
     if (meam) {
-      const spl_t* g = &c->spl[c->fn_off[4] + c->types[a]];
       for (long long p = b; p < e - 1; ++p)
         for (long long q = p + 1; q < e; ++q) {
           const pair_t* P = &c->pairs[p];
           const pair_t* Q = &c->pairs[q];
+          const spl_t* g = g_spline(c, c->types[a], c->types[P->j], c->types[Q->j]);
           double cs = P->dist[0] * Q->dist[0] + P->dist[1] * Q->dist[1] + P->dist[2] * Q->dist[2];
           double gv;
-          if (c->style == ORC_MEAM_SPLINE) {
+          if (is_spline_style(c)) {
             double sh;
             int kn = spl_knotshift(g, cs, &sh);
             if (kn == -1) continue;

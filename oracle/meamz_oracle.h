@@ -14,10 +14,11 @@
 extern "C" {
 #endif
 
-enum { ORC_EAM_SPLINE = 0, ORC_MEAM_SPLINE = 1, ORC_EAM = 2, ORC_MEAM = 3 };
+enum { ORC_EAM_SPLINE = 0, ORC_MEAM_SPLINE = 1, ORC_EAM = 2, ORC_MEAM = 3, ORC_GMEAM_SPLINE = 4, ORC_GMEAM = 5 };
 
 /* One potential's spline layout.  Splines are listed in file order
- * (phi Pair_ij, rho Pair_j, U Pair_i [, f Pair_ij, g Triplet_i]).
+ * (phi Pair_ij, rho Pair_j, U Pair_i [, f Pair_ij, g Triplet_i; gmeam styles:
+ * g Triplet_i_jk, ntypes * ntypes (ntypes + 1) / 2 splines]).
  * bc holds two ints per spline: -1 natural, 0 free slope, 1 fixed slope. */
 typedef struct {
   int style;

@@ -187,6 +187,31 @@ def main():
     _finish("meam_edges", pot, db, truth=truth, fd=1e-6,
             note="5 atoms: bonds at exactly the f cutoff, collinear triplet, g on [-0.5,0.5]")
 
+    gmeam_cases()
+
+
+def gmeam_cases():
+    """gmeam styles: the g spline of a triplet is chosen by the type of i and the
+    pair of types of j and k (Triplet_i_jk), and every f spline carries its own
+    f-max penalty term.  ``python tests/golden/make_golden.py gmeam`` writes only
+    these cases."""
+    rng = np.random.Generator(np.random.PCG64(5150))
+    w = gen.workload(2, scale=8 / 512)
+    small = [c for c in range(w.db.ncells) if w.db.natoms[c] <= 200][:6]
+    nk = dict(phi=10, rho=10, U=10, f=10, g=10)
+    pot = gen.make_potential(rng, "gmeam/spline", 2, 5.0, nk, (-1.0, 3.0))
+    truth = gen.make_potential(rng, "gmeam/spline", 2, 5.0, nk, (-1.0, 3.0))
+    _finish("gmeam_spline_binary", pot, _slice(w, small[:4]), truth=truth, fd=1e-6,
+            note="binary gmeam/spline (6 g splines by i, jk), FD gradient")
+    pot = pot.copy()
+    pot.style = "gmeam"
+    pot.splines[8].xn = 4.2          # one f spline (f_AB) ends before the phi cutoff
+    _finish("gmeam_point_binary", pot, _slice(w, small[:3]), truth=truth,
+            note="binary gmeam (point style), f_AB on [1.8,4.2]")
+
 
 if __name__ == "__main__":
-    main()
+    if len(sys.argv) > 1 and sys.argv[1] == "gmeam":
+        gmeam_cases()
+    else:
+        main()
