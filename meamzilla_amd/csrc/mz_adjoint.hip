@@ -60,6 +60,7 @@ struct AdjArgs {
   int nrad;                // interval records below the g records (phi, rho, U, f)
   int rrep_log2;           // log2 of the lane replicas of their adjoints, -1: none
   int rrep_stride;         // doubles per replica (nrad NC padded to 1 mod 32: replicas on distinct banks)
+  int nchunks, cpw;        // chunks, and consecutive chunks per workgroup
 };
 
 struct ChainArgs {
@@ -195,10 +196,39 @@ __global__ void __launch_bounds__(kAdjBlock) k_adjoint(const AdjArgs A) {
   constexpr int NC = Q ? 6 : 4;  // coefficients per g interval that carry adjoints
   const int cand = blockIdx.y;
   const int tid = threadIdx.x;
-  const Chunk ch = A.chunks[blockIdx.x];
   const int ncoef = A.pot.ncoef;
   constexpr int R = GR;
   const int nt = A.pot.nt;
+
+  // The candidate's spline table stays in global memory (cache-resident; it
+  // is read per pair and per atom, not per triplet), so LDS goes to the
+  // accumulators; only the g coefficients, read per triplet, are staged.
+  const char* tbl = A.tables + (size_t)cand * A.table_bytes;
+  const SplHead* HD = reinterpret_cast<const SplHead*>(tbl);
+  const SplInterval* IV = reinterpret_cast<const SplInterval*>(tbl + sizeof(SplHead) * A.pot.nspl);
+  const int ghb = GP ? A.pot.ng * (int)sizeof(SplHead) : 0;       // g heads (GP)
+  const SplHead* GH = reinterpret_cast<const SplHead*>(dyn);
+  double* gt = reinterpret_cast<double*>(dyn + ghb);              // g coefficients, [record][NC]
+  double* atab = gt + (MEAM ? A.ngco * NC : 0);                   // coefficient + head adjoints
+  double* gacc = atab + A.ntab;                                   // [g record][coef][replica]
+  if (GP) {
+    const int4* hsrc = reinterpret_cast<const int4*>(tbl + sizeof(SplHead) * A.pot.fn_off[4]);
+    for (int q = tid; q < ghb / 16; q += kAdjBlock) reinterpret_cast<int4*>(dyn)[q] = hsrc[q];
+  }
+  if (MEAM) {
+    const SplInterval* src = reinterpret_cast<const SplInterval*>(A.tables + (size_t)cand * A.table_bytes +
+                                                                  sizeof(SplHead) * A.pot.nspl);
+    for (int q = tid; q < A.ngco * NC; q += kAdjBlock) gt[q] = src[A.gco0 + q / NC].c[q % NC];
+  }
+  const int nacc = A.ntab + (MEAM ? A.ngco * NC * R : 0);
+  for (int q = tid; q < nacc; q += kAdjBlock) atab[q] = 0.0;
+
+  // A workgroup takes cpw consecutive chunks of one candidate (similar work in
+  // the decreasing-work order): table staging, zeroing, the g-replica
+  // reduction and the slot flush are paid once per workgroup.
+  const int cbeg = blockIdx.x * A.cpw, cend = min(cbeg + A.cpw, A.nchunks);
+  for (int ci = cbeg; ci < cend; ++ci) {
+  const Chunk ch = A.chunks[ci];
   const int p = tid;
   const bool has = p < ch.np;
   const int ba = tid >> 2;  // atom of this lane in phase B
@@ -252,28 +282,6 @@ __global__ void __launch_bounds__(kAdjBlock) k_adjoint(const AdjArgs A) {
     ti = A.types[ig];
     tj = A.types[j];
   }
-  // The candidate's spline table stays in global memory (cache-resident; it
-  // is read per pair and per atom, not per triplet), so LDS goes to the
-  // accumulators; only the g coefficients, read per triplet, are staged.
-  const char* tbl = A.tables + (size_t)cand * A.table_bytes;
-  const SplHead* HD = reinterpret_cast<const SplHead*>(tbl);
-  const SplInterval* IV = reinterpret_cast<const SplInterval*>(tbl + sizeof(SplHead) * A.pot.nspl);
-  const int ghb = GP ? A.pot.ng * (int)sizeof(SplHead) : 0;       // g heads (GP)
-  const SplHead* GH = reinterpret_cast<const SplHead*>(dyn);
-  double* gt = reinterpret_cast<double*>(dyn + ghb);              // g coefficients, [record][NC]
-  double* atab = gt + (MEAM ? A.ngco * NC : 0);                   // coefficient + head adjoints
-  double* gacc = atab + A.ntab;                                   // [g record][coef][replica]
-  if (GP) {
-    const int4* hsrc = reinterpret_cast<const int4*>(tbl + sizeof(SplHead) * A.pot.fn_off[4]);
-    for (int q = tid; q < ghb / 16; q += kAdjBlock) reinterpret_cast<int4*>(dyn)[q] = hsrc[q];
-  }
-  if (MEAM) {
-    const SplInterval* src = reinterpret_cast<const SplInterval*>(A.tables + (size_t)cand * A.table_bytes +
-                                                                  sizeof(SplHead) * A.pot.nspl);
-    for (int q = tid; q < A.ngco * NC; q += kAdjBlock) gt[q] = src[A.gco0 + q / NC].c[q % NC];
-  }
-  const int nacc = A.ntab + (MEAM ? A.ngco * NC * R : 0);
-  for (int q = tid; q < nacc; q += kAdjBlock) atab[q] = 0.0;
   for (int a = tid; a <= ch.na; a += kAdjBlock) poff[a] = A.atom_poff[ch.a0 + a] - ch.p0;
   __syncthreads();
 
@@ -474,6 +482,9 @@ __global__ void __launch_bounds__(kAdjBlock) k_adjoint(const AdjArgs A) {
     for (int rep = 0; rep < RR; ++rep) sacc += rrep[rep * RS + q];
     atab[(q / NC) * 6 + q % NC] += sacc;
   }
+  // The next chunk overwrites the partner-data area only after its first
+  // barrier, which every lane reaches after this reduction.
+  }  // chunks
   if (MEAM) {
     for (int q = tid; q < A.ngco * NC; q += kAdjBlock) {
       double sacc = 0.0;
@@ -816,7 +827,7 @@ int ensure_adjoint_buffers(mz_ctx* c, int ncand) {
 
 template <bool M, bool P, bool Q, bool GP>
 void launch_adjoint_g(hipStream_t st, const mz::AdjArgs& a, int nchunks, int ncand, int dyn) {
-  const dim3 grid(nchunks, ncand), block(mz::kAdjBlock);
+  const dim3 grid((nchunks + a.cpw - 1) / a.cpw, ncand), block(mz::kAdjBlock);
   if (a.grep_log2 == 5) hipLaunchKernelGGL((mz::k_adjoint<M, P, Q, 32, GP>), grid, block, dyn, st, a);
   else if (a.grep_log2 == 3) hipLaunchKernelGGL((mz::k_adjoint<M, P, Q, 8, GP>), grid, block, dyn, st, a);
   else hipLaunchKernelGGL((mz::k_adjoint<M, P, Q, 1, GP>), grid, block, dyn, st, a);
@@ -824,7 +835,9 @@ void launch_adjoint_g(hipStream_t st, const mz::AdjArgs& a, int nchunks, int nca
 
 template <bool M, bool P, bool Q>
 void launch_adjoint_t(hipStream_t st, const mz::AdjArgs& a, int nchunks, int ncand, int dyn) {
-  if (!M) hipLaunchKernelGGL((mz::k_adjoint<M, P, Q, 1, false>), dim3(nchunks, ncand), dim3(mz::kAdjBlock), dyn, st, a);
+  if (!M)
+    hipLaunchKernelGGL((mz::k_adjoint<M, P, Q, 1, false>), dim3((nchunks + a.cpw - 1) / a.cpw, ncand),
+                       dim3(mz::kAdjBlock), dyn, st, a);
   else if (a.pot.gpair) launch_adjoint_g<M, P, Q, true>(st, a, nchunks, ncand, dyn);
   else launch_adjoint_g<M, P, Q, false>(st, a, nchunks, ncand, dyn);
 }
@@ -873,6 +886,11 @@ static int run_grad(mz_ctx* c, int ncand) {
   a.nrad = X.nrad;
   a.rrep_log2 = X.rrep_log2;
   a.rrep_stride = X.rrep_stride;
+  a.nchunks = X.nchunks;
+  // two chunks per workgroup while the grid keeps >= 8 rounds of 1024
+  // workgroups (measured: 1, 2, 4, 8 chunks -> 5.30, 4.98, 5.23, 5.19 ms on
+  // the one-element 256-cell workload at 64 candidates)
+  a.cpw = (long long)X.nchunks * ncand >= 2LL * 8 * 1024 ? 2 : 1;
   const bool q = c->pot.quintic != 0;
   if (c->meam) {
     if (c->point) { if (q) launch_adjoint_t<true, true, true>(st, a, X.nchunks, ncand, X.dyn_bytes);
