@@ -289,6 +289,8 @@ __global__ void __launch_bounds__(kAdjBlock) k_adjoint(const AdjArgs A) {
   const int ia = ig - ch.a0;
   int pij = 0;
   double r = 0, invr = 0, ux = 0, uy = 0, uz = 0, fj = 0, dfj = 0, drho = 0, aj = 0, wx = 0, wy = 0, wz = 0;
+  int rrec = -1, frec = -1;  // located rho and f interval records (knot style)
+  double trho = 0, tf = 0, rinv = 0, finv = 0;
   if (has) {
     r = __dsqrt_rn(__dadd_rn(__dadd_rn(__dmul_rn(d[0], d[0]), __dmul_rn(d[1], d[1])), __dmul_rn(d[2], d[2])));
     invr = 1.0 / r;
@@ -297,10 +299,33 @@ __global__ void __launch_bounds__(kAdjBlock) k_adjoint(const AdjArgs A) {
     uz = d[2] / r;
     pij = pair_ij(ti, tj, nt);
     double v, dv;
-    if (eval_radial<POINT, Q>(HD[A.pot.fn_off[1] + tj], IV, r, v, dv)) drho = dv;
-    if (MEAM && eval_radial<POINT, Q>(HD[A.pot.fn_off[3] + pij], IV, r, v, dv)) {
-      fj = v;
-      dfj = dv;
+    if (POINT) {
+      if (eval_radial<POINT, Q>(HD[A.pot.fn_off[1] + tj], IV, r, v, dv)) drho = dv;
+      if (MEAM && eval_radial<POINT, Q>(HD[A.pot.fn_off[3] + pij], IV, r, v, dv)) {
+        fj = v;
+        dfj = dv;
+      }
+    } else {
+      // knot style: the located intervals are kept for the coefficient
+      // adjoints of phase D
+      const SplHead& Hr = HD[A.pot.fn_off[1] + tj];
+      int k;
+      if (locate_knot(Hr, IV + Hr.coef_off, r, k, trho)) {
+        rrec = Hr.coef_off + k;
+        rinv = Hr.invstep;
+        poly<Q>(IV[rrec], trho, rinv, v, dv);
+        drho = dv;
+      }
+      if (MEAM) {
+        const SplHead& Hf = HD[A.pot.fn_off[3] + pij];
+        if (locate_knot(Hf, IV + Hf.coef_off, r, k, tf)) {
+          frec = Hf.coef_off + k;
+          finv = Hf.invstep;
+          poly<Q>(IV[frec], tf, finv, v, dv);
+          fj = v;
+          dfj = dv;
+        }
+      }
     }
     const double vx = mxx * ux + mzx * uz, vy = myy * uy + mxy * ux, vz = mzz * uz + myz * uy;
     wx = lw[0] - r * vx;
@@ -470,10 +495,15 @@ __global__ void __launch_bounds__(kAdjBlock) k_adjoint(const AdjArgs A) {
     const double ps = psi[ia], up = Up[ia];
     const int sphi = A.pot.fn_off[0] + pij, srho = A.pot.fn_off[1] + tj;
     acc_radial<POINT, Q>(atab, sk, ncoef, sphi, HD[sphi], IV, r, 0.5 * lamE, 0.5 * aj);
-    acc_radial<POINT, Q>(atab, sk, ncoef, srho, HD[srho], IV, r, ps, aj * up);
-    if (MEAM) {
-      const int sf = A.pot.fn_off[3] + pij;
-      acc_radial<POINT, Q>(atab, sk, ncoef, sf, HD[sf], IV, r, Phi, Phip);
+    if (POINT) {
+      acc_radial<POINT, Q>(atab, sk, ncoef, srho, HD[srho], IV, r, ps, aj * up);
+      if (MEAM) {
+        const int sf = A.pot.fn_off[3] + pij;
+        acc_radial<POINT, Q>(atab, sk, ncoef, sf, HD[sf], IV, r, Phi, Phip);
+      }
+    } else {
+      if (rrec >= 0) acc_interval<Q>(sk, rrec, trho, ps, aj * up * rinv);
+      if (MEAM && frec >= 0) acc_interval<Q>(sk, frec, tf, Phi, Phip * finv);
     }
   }
   __syncthreads();
