@@ -103,6 +103,38 @@ def test_adjoint_ternary_matches_central_differences():
     assert _rel(g[idx], gcd) < GTOL, _rel(g[idx], gcd)
 
 
+def test_gmeam_ternary_matches_oracle_and_central_differences():
+    """gmeam/spline with three elements (18 g splines chosen per triplet by the
+    type of i and the pair of types j, k): objective, residuals and forces
+    against the oracle, batched against single evaluation, and the adjoint
+    against central differences on a seeded sample of coefficients."""
+    _say("gmeam ternary")
+    rng = np.random.Generator(np.random.PCG64(91))
+    w = gen.workload(2, scale=8 / 512)
+    db = w.db.subset([c for c in range(w.db.ncells) if w.db.natoms[c] <= 200][:3])
+    db.types = rng.integers(0, 3, size=db.total_atoms).astype(np.int32)
+    nk = dict(phi=10, rho=10, U=10, f=10, g=10)
+    pot = gen.make_potential(rng, "gmeam/spline", 3, 5.0, nk, (-1.0, 3.0))
+    truth = gen.make_potential(rng, "gmeam/spline", 3, 5.0, nk, (-1.0, 3.0))
+    e, s, f = pyoracle.evaluator(truth, db)
+    db.energy0, db.stress0, db.force0 = e, s, f
+    o = pyoracle.Oracle(pot, db)
+    obj = Objective(pot, db)
+    ref = o.eval()
+    r = obj.compute()
+    assert abs(r["F"] - ref["F"]) <= 1e-10 * abs(ref["F"])
+    for k in ("errvec", "forces"):
+        assert _rel(r[k], ref[k]) < 1e-10, k
+    kv = pot.knot_vector()
+    F, g = obj.objective_grad(kv)
+    Fb, Gb = obj.batch(np.stack([kv, kv]), grad=True)
+    assert np.allclose(Fb, F, rtol=1e-12) and _rel(Gb[1], g) < 1e-12
+    slots = np.asarray(pot.coeff_slots())
+    idx = np.sort(np.random.default_rng(5).choice(slots, size=40, replace=False))
+    gcd = _cd(obj.objective, kv, idx)
+    assert _rel(g[idx], gcd) < GTOL, _rel(g[idx], gcd)
+
+
 def test_batched_gradient_matches_single():
     _say("batched gradient")
     w = gen.workload(2, scale=12 / 512)
