@@ -183,7 +183,7 @@ __device__ __forceinline__ GSpl gspl_of(const SplHead& H, int gco0) {
 // GR: lane replicas of the g-coefficient adjoints (32 keeps the 32 lanes that
 // one LDS cycle serves on distinct replicas; fewer when LDS is short).
 // GP: the g spline of a triplet depends on both neighbours' types (gmeam),
-// chosen per triplet from heads staged in LDS.
+// chosen per triplet from the heads staged in LDS.
 template <bool MEAM, bool POINT, bool Q, int GR, bool GP>
 __global__ void __launch_bounds__(kAdjBlock) k_adjoint(const AdjArgs A) {
   extern __shared__ __attribute__((aligned(64))) char dyn[];
@@ -200,21 +200,19 @@ __global__ void __launch_bounds__(kAdjBlock) k_adjoint(const AdjArgs A) {
   constexpr int R = GR;
   const int nt = A.pot.nt;
 
-  // The candidate's spline table stays in global memory (cache-resident; it
-  // is read per pair and per atom, not per triplet), so LDS goes to the
-  // accumulators; only the g coefficients, read per triplet, are staged.
+  // The candidate's interval records stay in global memory (cache-resident;
+  // read per pair and per atom, not per triplet), so LDS goes to the
+  // accumulators; staged are the spline heads (every evaluation starts from
+  // one) and the g coefficients, read per triplet.
   const char* tbl = A.tables + (size_t)cand * A.table_bytes;
-  const SplHead* HD = reinterpret_cast<const SplHead*>(tbl);
   const SplInterval* IV = reinterpret_cast<const SplInterval*>(tbl + sizeof(SplHead) * A.pot.nspl);
-  const int ghb = GP ? A.pot.ng * (int)sizeof(SplHead) : 0;       // g heads (GP)
-  const SplHead* GH = reinterpret_cast<const SplHead*>(dyn);
+  const int ghb = A.pot.nspl * (int)sizeof(SplHead);              // all spline heads, staged
+  const SplHead* HD = reinterpret_cast<const SplHead*>(dyn);
+  const SplHead* GH = HD + A.pot.fn_off[4];
   double* gt = reinterpret_cast<double*>(dyn + ghb);              // g coefficients, [record][NC]
   double* atab = gt + (MEAM ? A.ngco * NC : 0);                   // coefficient + head adjoints
   double* gacc = atab + A.ntab;                                   // [g record][coef][replica]
-  if (GP) {
-    const int4* hsrc = reinterpret_cast<const int4*>(tbl + sizeof(SplHead) * A.pot.fn_off[4]);
-    for (int q = tid; q < ghb / 16; q += kAdjBlock) reinterpret_cast<int4*>(dyn)[q] = hsrc[q];
-  }
+  for (int q = tid; q < ghb / 16; q += kAdjBlock) reinterpret_cast<int4*>(dyn)[q] = reinterpret_cast<const int4*>(tbl)[q];
   if (MEAM) {
     const SplInterval* src = reinterpret_cast<const SplInterval*>(A.tables + (size_t)cand * A.table_bytes +
                                                                   sizeof(SplHead) * A.pot.nspl);
@@ -816,8 +814,7 @@ int setup_adjoint(mz_ctx* c) {
   // limit: replicas (fewer lanes adding into one address) were measured to
   // matter more than workgroups per CU.
   const int nc_g = c->pot.quintic ? 6 : 4;
-  const int fixed = (X.ngco * nc_g + X.ntab) * (int)sizeof(double) +
-                    (c->pot.gpair ? c->pot.ng * (int)sizeof(mz::SplHead) : 0);
+  const int fixed = (X.ngco * nc_g + X.ntab) * (int)sizeof(double) + c->nspl * (int)sizeof(mz::SplHead);
   const int stat = (int)(mz::kPartnerFields * sizeof(double) * mz::kAdjBlock + 2 * sizeof(double) * mz::kAdjBlock +
                          2 * sizeof(double) * mz::kAdjAtoms + sizeof(int) * (mz::kAdjAtoms + 1));
   X.grep_log2 = -1;
