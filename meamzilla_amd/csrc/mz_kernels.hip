@@ -347,13 +347,13 @@ __global__ void __launch_bounds__(BLOCK) k_atoms(const AtomArgs args) {
     cand[c] = valid[c] ? cc : args.ncand - 1;
   }
 
-  // LDS per candidate: its spline table (unless TG; then the table is read
-  // from global memory, cache-resident, per pair and per atom), the g splines'
-  // heads (TG only; otherwise they are part of the staged table), then the g
-  // intervals again as compact power-of-two records (c0..c3, or c0..c5 padded
-  // to 8 doubles), which the triplet walk reads per triplet.
+  // LDS per candidate: its spline table (with TG only the heads; the
+  // interval records are then read from global memory, cache-resident, per
+  // pair and per atom), then the g intervals again as compact power-of-two
+  // records (c0..c3, or c0..c5 padded to 8 doubles), which the triplet walk
+  // reads per triplet.
   const int gbytes = MEAM ? args.pot.ngco * NG * 8 : 0;
-  const int ghb = MEAM && TG ? args.pot.ng * (int)sizeof(SplHead) : 0;
+  const int ghb = TG ? args.pot.nspl * (int)sizeof(SplHead) : 0;
   const int tbytes = TG ? 0 : TB;
   const int cbytes = tbytes + ghb + gbytes;
 #pragma unroll
@@ -364,24 +364,23 @@ __global__ void __launch_bounds__(BLOCK) k_atoms(const AtomArgs args) {
       const int4* src = reinterpret_cast<const int4*>(tb);
       for (int q = tid; q < TB / 16; q += BLOCK) reinterpret_cast<int4*>(cb)[q] = src[q];
     }
+    if (TG) {
+      const int4* hsrc = reinterpret_cast<const int4*>(tb);
+      for (int q = tid; q < ghb / 16; q += BLOCK) reinterpret_cast<int4*>(cb)[q] = hsrc[q];
+    }
     if (MEAM) {
-      if (TG) {
-        const int4* hsrc = reinterpret_cast<const int4*>(tb + sizeof(SplHead) * args.pot.fn_off[4]);
-        for (int q = tid; q < ghb / 16; q += BLOCK) reinterpret_cast<int4*>(cb)[q] = hsrc[q];
-      }
       const SplInterval* gsrc = reinterpret_cast<const SplInterval*>(tb + sizeof(SplHead) * args.pot.nspl);
       double* gdst = reinterpret_cast<double*>(cb + tbytes + ghb);
       for (int q = tid; q < args.pot.ngco * NG; q += BLOCK)
         gdst[q] = (q % NG) < 6 ? gsrc[args.pot.gco0 + q / NG].c[q % NG] : 0.0;
     }
   }
-  auto HDc = [&](int c) {
-    return reinterpret_cast<const SplHead*>(TG ? args.tables + (size_t)cand[c] * TB : dyn + (size_t)c * cbytes);
+  auto HDc = [&](int c) { return reinterpret_cast<const SplHead*>(dyn + (size_t)c * cbytes); };
+  auto IVc = [&](int c) {
+    return reinterpret_cast<const SplInterval*>((TG ? args.tables + (size_t)cand[c] * TB : dyn + (size_t)c * cbytes) +
+                                                sizeof(SplHead) * args.pot.nspl);
   };
-  auto IVc = [&](int c) { return reinterpret_cast<const SplInterval*>(HDc(c) + args.pot.nspl); };
-  auto GHc = [&](int c) {  // g heads by element type (by type and pair of types with GP)
-    return TG ? reinterpret_cast<const SplHead*>(dyn + (size_t)c * cbytes) : HDc(c) + args.pot.fn_off[4];
-  };
+  auto GHc = [&](int c) { return HDc(c) + args.pot.fn_off[4]; };  // g heads by element type (type, pair of types with GP)
   auto GTc = [&](int c) { return reinterpret_cast<const double*>(dyn + (size_t)c * cbytes + tbytes + ghb); };
   for (int a = tid; a <= ch.na; a += BLOCK) sm.poff[a] = args.atom_poff[ch.a0 + a] - ch.p0;
   __syncthreads();
@@ -838,7 +837,7 @@ bool atoms_tables_global(const PotDesc& pot, int ct) {
 size_t atoms_dyn_lds(const PotDesc& pot, int ct) {
   const bool meam = pot.ngco > 0;
   const size_t g = meam ? (size_t)pot.ngco * 8 * g_record_doubles(pot.quintic != 0) : 0;
-  if (atoms_tables_global(pot, ct)) return (size_t)ct * ((size_t)pot.ng * sizeof(SplHead) + g);
+  if (atoms_tables_global(pot, ct)) return (size_t)ct * ((size_t)pot.nspl * sizeof(SplHead) + g);
   return (size_t)ct * ((size_t)pot.table_bytes + g);
 }
 
