@@ -12,7 +12,9 @@
  * thread per context.  All doubles are IEEE binary64.
  *
  * Knot vector.  A potential is a list of splines in file order (phi Pair_ij,
- * rho Pair_j, U Pair_i and, for MEAM, f Pair_ij and g Triplet_i).  Each spline
+ * rho Pair_j, U Pair_i and, for MEAM, f Pair_ij and g Triplet_i; pair styles
+ * hold phi Pair_ij alone and their residual vector has no density or penalty
+ * rows).  Each spline
  * contributes n+4 doubles: its n knot values y_0..y_{n-1}, then grad_L, grad_R,
  * grad2_L, grad2_R (boundary slopes and, for quintic splines, boundary second
  * derivatives).  Every evaluation entry point takes such a vector.
@@ -39,6 +41,8 @@ extern "C" {
 #define MZ_STYLE_MEAM 3        /* "meam" (point-evaluated) */
 #define MZ_STYLE_GMEAM_SPLINE 4 /* "gmeam/spline": g indexed by (type i, types j k) */
 #define MZ_STYLE_GMEAM 5       /* "gmeam" (point-evaluated) */
+#define MZ_STYLE_PAIR_SPLINE 6 /* "pair/spline": phi only (src/pots/pot_pair_spline/pot_pair_spline.cpp:74-141) */
+#define MZ_STYLE_PAIR 7        /* "pair" (point-evaluated, src/pots/pot_pair/pot_pair.cpp:73-144) */
 
 /* Spline layout of one potential.  bc holds two ints per spline:
  * -1 natural, 0 free slope, 1 fixed slope (the "#B" constraint line). */
@@ -104,12 +108,15 @@ const char* mz_last_error(void);
  * src/pots/pot_meam_spline/pot_meam_spline.cpp:56-283, PotEAMSpline
  * src/pots/pot_eam_spline/pot_eam_spline.cpp:43-224, PotMEAM::compute
  * src/pots/pot_meam/pot_meam.cpp:95-275, PotEAM::compute
- * src/pots/pot_eam/pot_eam.cpp:129-251, with accumulate_error
+ * src/pots/pot_eam/pot_eam.cpp:129-251, PotPairSpline / PotPair
+ * src/pots/pot_pair_spline/pot_pair_spline.cpp:74-141,
+ * src/pots/pot_pair/pot_pair.cpp:73-144, with accumulate_error
  * src/potential.cpp:415-491. */
 int mz_objective(mz_ctx* ctx, const double* knots, double* F, double* errvec, double* cell_energy,
                  double* cell_stress, double* forces);
 
-/* Per-atom densities before clamping.
+/* Per-atom densities before clamping (MZ_ERR_LAYOUT for the pair styles,
+ * which have none).
  * Replaces: compute_densities src/pots/pot_meam_spline/pot_meam_spline.cpp:79-96,289-349. */
 int mz_densities(mz_ctx* ctx, const double* knots, double* dens);
 

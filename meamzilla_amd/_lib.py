@@ -17,7 +17,8 @@ from .formats import Database, PotentialSpec
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libmeamz.so")
 HEADER = os.path.join(os.path.dirname(HERE), "include", "meamz.h")
-STYLE_CODE = {"eam/spline": 0, "meam/spline": 1, "eam": 2, "meam": 3, "gmeam/spline": 4, "gmeam": 5}
+STYLE_CODE = {"eam/spline": 0, "meam/spline": 1, "eam": 2, "meam": 3, "gmeam/spline": 4, "gmeam": 5,
+              "pair/spline": 6, "pair": 7}
 
 
 class MeamzError(RuntimeError):

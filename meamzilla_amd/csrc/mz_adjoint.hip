@@ -67,6 +67,7 @@ struct ChainArgs {
   int nspl, ncoef, ntab, nknot, meam, nt;
   int f0, nf, u0;
   int fper;                // one f-max penalty per f spline (gmeam)
+  int nodens;              // pair styles: no U'(mid) penalty
   const int* nknots;
   const int* basis;
   const int* koff;
@@ -297,7 +298,9 @@ __global__ void __launch_bounds__(kAdjBlock) k_adjoint(const AdjArgs A) {
     uz = d[2] / r;
     pij = pair_ij(ti, tj, nt);
     double v, dv;
-    if (POINT) {
+    if (A.pot.nodens) {
+      // pair styles: phi only, nothing to locate here
+    } else if (POINT) {
       if (eval_radial<POINT, Q>(HD[A.pot.fn_off[1] + tj], IV, r, v, dv)) drho = dv;
       if (MEAM && eval_radial<POINT, Q>(HD[A.pot.fn_off[3] + pij], IV, r, v, dv)) {
         fj = v;
@@ -355,7 +358,10 @@ __global__ void __launch_bounds__(kAdjBlock) k_adjoint(const AdjArgs A) {
     }
   Sa += dpp_move<kDppXor1>(Sa);
   Sa += dpp_move<kDppXor2>(Sa);
-  if (ba < ch.na && (tid & 3) == 0) {
+  if (ba < ch.na && (tid & 3) == 0 && A.pot.nodens) {
+    psi[ba] = 0.0;  // pair styles: no density, no embedding
+    Up[ba] = 0.0;
+  } else if (ba < ch.na && (tid & 3) == 0) {
     const int a = ba;
     const double rho = rho_b;
     const int su = A.pot.fn_off[2] + type_b;
@@ -494,7 +500,7 @@ __global__ void __launch_bounds__(kAdjBlock) k_adjoint(const AdjArgs A) {
     const int sphi = A.pot.fn_off[0] + pij, srho = A.pot.fn_off[1] + tj;
     acc_radial<POINT, Q>(atab, sk, ncoef, sphi, HD[sphi], IV, r, 0.5 * lamE, 0.5 * aj);
     if (POINT) {
-      acc_radial<POINT, Q>(atab, sk, ncoef, srho, HD[srho], IV, r, ps, aj * up);
+      if (!A.pot.nodens) acc_radial<POINT, Q>(atab, sk, ncoef, srho, HD[srho], IV, r, ps, aj * up);
       if (MEAM) {
         const int sf = A.pot.fn_off[3] + pij;
         acc_radial<POINT, Q>(atab, sk, ncoef, sf, HD[sf], IV, r, Phi, Phip);
@@ -549,7 +555,7 @@ __global__ void __launch_bounds__(64) k_chain(const ChainArgs A) {
     const SplInterval* IV = reinterpret_cast<const SplInterval*>(tb + sizeof(SplHead) * A.nspl);
     const SplHead& H = HD[s];
     const double* kv = A.knots + (size_t)cand * A.nknot;
-    if (!A.meam && s >= A.u0 && s < A.u0 + A.nt) {
+    if (!A.meam && !A.nodens && s >= A.u0 && s < A.u0 + A.nt) {
       // U'(midpoint) penalty, src/pots/pot_eam_spline/pot_eam_spline.cpp:211-218
       const double mid = 0.5 * (H.x0 + H.xn);
       double d, d2;
@@ -943,6 +949,7 @@ static int run_grad(mz_ctx* c, int ncand) {
   ca.nf = c->nt * (c->nt + 1) / 2;
   ca.fper = c->meam && c->nfpen == ca.nf && (c->style == mz::kGMeamSpline || c->style == mz::kGMeam);
   ca.u0 = c->pot.fn_off[2];
+  ca.nodens = c->pot.nodens;
   ca.nknots = c->d_nknots.p;
   ca.basis = c->d_basis.p;
   ca.koff = c->d_koff.p;

@@ -11,7 +11,16 @@ constexpr int kMaxSplines = 96;      // splines per potential
 constexpr int kMaxImage = 127;       // |image index| per direction (packed as int8)
 constexpr double kDummyWeight = 100.0;  // DUMMY_WEIGHT, src/pots/pot_pair/pot_pair.h:39
 
-enum Style : int { kEamSpline = 0, kMeamSpline = 1, kEam = 2, kMeam = 3, kGMeamSpline = 4, kGMeam = 5 };
+enum Style : int {
+  kEamSpline = 0,
+  kMeamSpline = 1,
+  kEam = 2,
+  kMeam = 3,
+  kGMeamSpline = 4,
+  kGMeam = 5,
+  kPairSpline = 6,
+  kPair = 7
+};
 
 // Per-spline header in the device table (16 doubles = 128 bytes).
 struct SplHead {
@@ -64,6 +73,7 @@ struct PotDesc {
   int32_t g_keep_all;   // every g spline's x_n (1 + 1e-8) exceeds any cosine: no triplet is cut
   int32_t gpair;        // g indexed by (type i, types j k) (gmeam styles, more than one type)
   int32_t ng;           // number of g splines (nt, or nt * nt (nt + 1) / 2 with gpair)
+  int32_t nodens;       // pair styles: phi only, no density, embedding or density penalty
 };
 
 // Doubles per g-interval record in the atom kernel's compact LDS copy

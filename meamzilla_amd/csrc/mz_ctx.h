@@ -81,7 +81,7 @@ struct AdjointExtra {
 struct mz_ctx {
   mz::PotDesc pot{};
   int style = 0, nt = 0, nspl = 0;
-  bool meam = false, point = false;
+  bool meam = false, point = false, pair = false;
   int nfpen = 0;  // f-max penalty terms: 1 (meam), one per f spline (gmeam)
   std::vector<int> basis, nknots, bc, koff, coef_off;
   std::vector<double> x0, xn;

@@ -45,17 +45,19 @@ int pair_ij_h(int ti, int tj, int nt) {
 namespace {
 
 int build_layout(mz_ctx* c, const mz_layout* L) {
-  if (!L || L->style < 0 || L->style > 5) return fail(MZ_ERR_LAYOUT, "unknown style");
+  if (!L || L->style < 0 || L->style > 7) return fail(MZ_ERR_LAYOUT, "unknown style");
   if (L->ntypes < 1 || L->ntypes > mz::kMaxTypes) return fail(MZ_ERR_LAYOUT, "ntypes out of range");
   c->style = L->style;
   c->nt = L->ntypes;
   const bool gmeam = L->style == mz::kGMeamSpline || L->style == mz::kGMeam;
   c->meam = (L->style == mz::kMeamSpline || L->style == mz::kMeam || gmeam);
-  c->point = (L->style == mz::kEam || L->style == mz::kMeam || L->style == mz::kGMeam);
+  c->point = (L->style == mz::kEam || L->style == mz::kMeam || L->style == mz::kGMeam || L->style == mz::kPair);
+  // pair styles: phi Pair_ij alone (src/pots/pot_pair/pot_pair.cpp:24-37)
+  c->pair = (L->style == mz::kPairSpline || L->style == mz::kPair);
   const int nt = c->nt, npij = nt * (nt + 1) / 2;
   // g: Triplet_i (meam), Triplet_i_jk (gmeam, src/pots/pot_gmeam/pot_gmeam.cpp:31-36)
   const int counts[5] = {npij, nt, nt, npij, gmeam ? nt * npij : nt};
-  const int nfn = c->meam ? 5 : 3;
+  const int nfn = c->meam ? 5 : (c->pair ? 1 : 3);
   int tot = 0;
   for (int f = 0; f < 5; ++f) c->pot.fn_off[f] = 0;
   for (int f = 0; f < nfn; ++f) {
@@ -105,6 +107,7 @@ int build_layout(mz_ctx* c, const mz_layout* L) {
   c->pot.ngco = c->meam ? coff - c->pot.gco0 : 0;
   c->pot.ng = c->meam ? counts[4] : 0;
   c->pot.gpair = gmeam && nt > 1;  // one type: gmeam is meam
+  c->pot.nodens = c->pair ? 1 : 0;
   c->nfpen = gmeam ? npij : (c->meam ? 1 : 0);
   // |cos| <= 1 + a few ulps, so a knot-style g spline ending at or beyond 1
   // never cuts a triplet
@@ -195,7 +198,7 @@ void cell_pairs(const mz_ctx* c, int cc, const double* box, const double* pos, d
             // check_pair: first-knot test and inside flags
             const int fns[3] = {c->pot.fn_off[0] + pij, c->pot.fn_off[1] + tj, c->pot.fn_off[3] + pij};
             bool inside = false;
-            for (int q = 0; q < (c->meam ? 3 : 2); ++q) {
+            for (int q = 0; q < (c->meam ? 3 : (c->pair ? 1 : 2)); ++q) {
               const Grid& g = c->grids[fns[q]];
               if (r < g.x[0]) {
                 char buf[200];
@@ -416,6 +419,7 @@ mz::ResidArgs resid_args(const mz_ctx* c) {
   r.cell_chunks = c->d_cell_chunks.p;
   r.nchunks = (int)c->chunks.size();
   r.partial = c->d_partial.p;
+  r.nodens = c->pot.nodens;
   return r;
 }
 
@@ -435,6 +439,7 @@ mz::FinalArgs final_args(const mz_ctx* c) {
   f.nf = c->nt * (c->nt + 1) / 2;
   f.fper = c->meam && c->nfpen == f.nf && (c->style == mz::kGMeamSpline || c->style == mz::kGMeam);
   f.u0 = c->pot.fn_off[2];
+  f.nodens = c->pot.nodens;
   f.koff = c->d_koff.p;
   f.nknots = c->d_nknots.p;
   f.knots = c->d_knots.p;
@@ -575,7 +580,9 @@ int mz_create(const mz_layout* layout, const mz_db* db, mz_ctx** out) {
     na += c->natoms_c[k];
   }
   c->natoms = na;
-  c->nerr = 7 * c->ncells + 3 * na + c->ncells + (c->meam ? c->nfpen : c->nt);
+  // pair styles add no constraint or penalty rows (their extra_error is empty,
+  // src/pots/pot_pair/pot_pair.cpp:136-137)
+  c->nerr = 7 * c->ncells + 3 * na + (c->pair ? 0 : c->ncells + (c->meam ? c->nfpen : c->nt));
   c->types.assign(db->types, db->types + na);
   c->cell_of.resize(na);
   for (int k = 0; k < c->ncells; ++k)
@@ -600,7 +607,7 @@ int mz_create(const mz_layout* layout, const mz_db* db, mz_ctx** out) {
   const int radial[3] = {0, 1, 3};
   const int nt = c->nt, npij = nt * (nt + 1) / 2;
   const int counts[5] = {npij, nt, nt, npij, nt};
-  for (int q = 0; q < (c->meam ? 3 : 2); ++q)
+  for (int q = 0; q < (c->meam ? 3 : (c->pair ? 1 : 2)); ++q)
     for (int s = c->pot.fn_off[radial[q]]; s < c->pot.fn_off[radial[q]] + counts[radial[q]]; ++s)
       max_rcut = std::max(max_rcut, c->xn[s]);
 
@@ -784,6 +791,7 @@ int mz_objective(mz_ctx* c, const double* knots, double* F, double* errvec, doub
 
 int mz_densities(mz_ctx* c, const double* knots, double* dens) {
   if (!c || !knots || !dens) return fail(MZ_ERR_ARG, "null argument");
+  if (c->pair) return fail(MZ_ERR_LAYOUT, "pair styles have no densities");
   int rc = ensure_eval_buffers(c, 1);
   if (rc) return rc;
   if (c->d_rho.alloc(c->natoms)) return fail(MZ_ERR_NOMEM, "allocation failed");

@@ -73,6 +73,7 @@ struct ResidArgs {
   double* partial;        // [ncand][nblocks]
   double* lam;            // optional [ncand][natoms*3]: dF/d(force) = 2 w (F - F0)
   double* cell_adj;       // optional [ncand][ncells*8]: dF/dE_raw, dF/dS_raw (6), dF/dP (constraint)
+  int nodens;             // pair styles: no constraint rows in the residual vector
 };
 
 struct FinalArgs {
@@ -82,6 +83,7 @@ struct FinalArgs {
   int ncells, natoms, nerr, nt, nspl, nknot;
   int f0, nf, u0;
   int fper;               // one f-max penalty per f spline (gmeam) instead of one over all
+  int nodens;             // pair styles: no penalty term
   const int* koff;
   const int* nknots;
   const double* knots;

@@ -458,7 +458,7 @@ __global__ void __launch_bounds__(BLOCK) k_atoms(const AtomArgs args) {
       double rv[CT];
 #pragma unroll
       for (int c = 0; c < CT; ++c) rv[c] = 0.0;
-      if (locate_radial<POINT>(HD0[srho], IV0, rr, k, t, reg)) {
+      if (!args.pot.nodens && locate_radial<POINT>(HD0[srho], IV0, rr, k, t, reg)) {
 #pragma unroll
         for (int c = 0; c < CT; ++c) eval_located<Q>(HDc(c)[srho], IVc(c), k, t, reg, rv[c], brho[s][c]);
       }
@@ -538,9 +538,9 @@ __global__ void __launch_bounds__(BLOCK) k_atoms(const AtomArgs args) {
   }
   __syncthreads();
   // per-atom densities from the staged pair contributions (two candidates at
-  // a time share one eight-lane pass)
+  // a time share one eight-lane pass); pair styles have none
 #pragma unroll
-  for (int c = 0; c < CT; c += 2) {
+  for (int c = 0; c < (args.pot.nodens ? 0 : CT); c += 2) {
     const int c2 = c + 1 < CT ? c + 1 : c;
     atom_sums3<BLOCK>(
         sm.poff, ch.na,
@@ -562,6 +562,11 @@ __global__ void __launch_bounds__(BLOCK) k_atoms(const AtomArgs args) {
 #pragma unroll
   for (int c = 0; c < CT; ++c) c_acc[c] = 0.0;
   for (int a = tid; a < ch.na; a += BLOCK) {
+    if (args.pot.nodens) {
+#pragma unroll
+      for (int c = 0; c < CT; ++c) sm.dF[c][a] = 0.0;
+      continue;
+    }
     const int su = args.pot.fn_off[2] + args.types[ch.a0 + a];
 #pragma unroll
     for (int c = 0; c < CT; ++c) {
@@ -718,7 +723,7 @@ __global__ void __launch_bounds__(BLOCK) k_resid(const ResidArgs a) {
     } else {
       er = acc;
       dadj = 2.0 * er;
-      if (ev) ev[7 * (size_t)a.ncells + 3 * (size_t)a.natoms + c] = er;
+      if (ev && !a.nodens) ev[7 * (size_t)a.ncells + 3 * (size_t)a.natoms + c] = er;
     }
     sq = er * er;
     if (adj) adj[l] = dadj;
@@ -740,7 +745,9 @@ __global__ void __launch_bounds__(256) k_final(const FinalArgs a) {
   double* ev = a.errvec ? a.errvec + (size_t)cand * a.nerr : nullptr;
   const size_t tail = 8 * (size_t)a.ncells + 3 * (size_t)a.natoms;
   const double* kv = a.knots + (size_t)cand * a.nknot;
-  if (a.meam && a.fper) {
+  if (a.nodens) {
+    // pair styles: no penalty term (src/pots/pot_pair/pot_pair.cpp:136-137)
+  } else if (a.meam && a.fper) {
     // one term per f spline (src/pots/pot_gmeam_spline/pot_gmeam_spline.cpp:275-280)
     for (int q = 0; q < a.nf; ++q) {
       const int s2 = a.f0 + q;

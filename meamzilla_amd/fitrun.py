@@ -41,7 +41,7 @@ def fix_gauge(pot) -> None:
     blows up; anchoring one knot per rho and f spline removes them."""
     counts = formats.function_counts(pot.style, pot.ntypes)
     off = [sum(counts[:i]) for i in range(len(counts))]
-    idx = list(range(off[1], off[1] + counts[1]))
+    idx = list(range(off[1], off[1] + counts[1])) if len(counts) > 1 else []
     if len(counts) > 3:
         idx += list(range(off[3], off[3] + counts[3]))
     for i in idx:

@@ -19,7 +19,7 @@ from typing import List, Optional, Sequence
 
 import numpy as np
 
-STYLES = ("eam/spline", "meam/spline", "eam", "meam", "gmeam/spline", "gmeam")
+STYLES = ("eam/spline", "meam/spline", "eam", "meam", "gmeam/spline", "gmeam", "pair/spline", "pair")
 
 
 def n_pair_ij(nt: int) -> int:
@@ -31,6 +31,12 @@ def is_meam_style(style: str) -> bool:
     return style.startswith(("meam", "gmeam"))
 
 
+def is_pair_style(style: str) -> bool:
+    """Pair-only styles: phi alone, no density or embedding
+    (src/pots/pot_pair/pot_pair.cpp:24-37)."""
+    return style.startswith("pair")
+
+
 def is_gmeam_style(style: str) -> bool:
     """Styles whose g splines are indexed by (type i, types j k) (Triplet_i_jk)."""
     return style.startswith("gmeam")
@@ -39,7 +45,10 @@ def is_gmeam_style(style: str) -> bool:
 def function_counts(style: str, nt: int) -> List[int]:
     """Alloy members per function in file order (src/pot_fns.cpp:280-310):
     phi Pair_ij, rho Pair_j, U Pair_i, f Pair_ij, g Triplet_i (gmeam styles:
-    Triplet_i_jk, nt * nt (nt + 1) / 2 members, src/pots/pot_gmeam/pot_gmeam.cpp:31-36)."""
+    Triplet_i_jk, nt * nt (nt + 1) / 2 members, src/pots/pot_gmeam/pot_gmeam.cpp:31-36;
+    pair styles: phi only)."""
+    if is_pair_style(style):
+        return [n_pair_ij(nt)]
     counts = [n_pair_ij(nt), nt, nt]
     if is_meam_style(style):
         counts += [n_pair_ij(nt), nt * n_pair_ij(nt) if is_gmeam_style(style) else nt]
@@ -114,7 +123,7 @@ class PotentialSpec:
 
     def max_rcut(self) -> float:
         counts = function_counts(self.style, self.ntypes)
-        radial = [0, 1] + ([3] if is_meam_style(self.style) else [])
+        radial = [0] if is_pair_style(self.style) else [0, 1] + ([3] if is_meam_style(self.style) else [])
         r = 0.0
         off = 0
         for fi, c in enumerate(counts):

@@ -176,6 +176,8 @@ def make_potential(rng, style: str, nt: int, rc: float, nk: dict, u_range, basis
     counts = function_counts(style, nt)
     spl: List[Spline] = []
     spl += [_radial(rng, 1.8, rc, nk["phi"], -0.2, 0.4, basis) for _ in range(counts[0])]
+    if len(counts) == 1:  # pair styles: phi only
+        return PotentialSpec(style, nt, spl)
     spl += [_radial(rng, 1.8, rc, nk["rho"], 0.0, rho_amp, basis) for _ in range(counts[1])]
     spl += [_flat(rng, u_range[0], u_range[1], nk["U"], -0.5, 0.5, basis) for _ in range(counts[2])]
     if is_meam_style(style):

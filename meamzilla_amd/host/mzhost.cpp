@@ -37,6 +37,8 @@ int style_code(const std::string& s) {
   if (s == "meam") return MZ_STYLE_MEAM;
   if (s == "gmeam/spline") return MZ_STYLE_GMEAM_SPLINE;
   if (s == "gmeam") return MZ_STYLE_GMEAM;
+  if (s == "pair/spline") return MZ_STYLE_PAIR_SPLINE;
+  if (s == "pair") return MZ_STYLE_PAIR;
   throw Error("invalid potential style \"" + s + "\"");
 }
 
@@ -59,6 +61,7 @@ int splines_for_style(const std::string& style, int nt) {
   const int npij = nt * (nt + 1) / 2;
   const int eam = npij + nt + nt;
   style_code(style);
+  if (style.compare(0, 4, "pair") == 0) return npij;  // phi Pair_ij only
   if (style.compare(0, 5, "gmeam") == 0) return eam + npij + nt * npij;  // g Triplet_i_jk
   return (style.compare(0, 4, "meam") == 0) ? eam + npij + nt : eam;
 }
@@ -1060,7 +1063,10 @@ void write_data_files(const std::string& base, Potential& pot, const ConfigDb& d
   ErrorVec ev;
   std::vector<double> ce, cs, fr, dens;
   pot.compute_outputs(&ev, &ce, &cs, &fr);
-  pot.compute_densities(&dens);
+  // pair styles write neither densities nor constraints: Potential's
+  // write_extras and write_punish do nothing (src/potential.cpp:713-725)
+  const bool pair = pot.style().compare(0, 4, "pair") == 0;
+  if (!pair) pot.compute_densities(&dens);
   const int nc = db.ncells;
   int natoms = 0;
   for (int c = 0; c < nc; ++c) natoms += db.natoms[c];
@@ -1068,7 +1074,7 @@ void write_data_files(const std::string& base, Potential& pot, const ConfigDb& d
   const bool meam = gmeam || pot.style().compare(0, 4, "meam") == 0;
   using std::setw;
   using std::setprecision;
-  {  // densities (src/pots/pot_eam/pot_eam.cpp:506-543)
+  if (!pair) {  // densities (src/pots/pot_eam/pot_eam.cpp:506-543)
     std::ofstream o = open_out(numbered(base, ".rho", num));
     o << setw(6) << std::right << "conf:" << setw(5) << "atom" << setw(8) << "type" << setw(15) << "rho" << "\n";
     for (int c = 0, a = 0; c < nc; ++c)
@@ -1151,7 +1157,7 @@ void write_data_files(const std::string& base, Potential& pot, const ConfigDb& d
     o << "Sum of stress errors = " << ss << "\n";
     if (npun) o << "Sum of punish errors = " << sp << "\n";
   }
-  {  // constraints (src/pots/pot_eam/pot_eam.cpp:439-488, src/pots/pot_meam/pot_meam.cpp:444-490)
+  if (!pair) {  // constraints (src/pots/pot_eam/pot_eam.cpp:439-488, src/pots/pot_meam/pot_meam.cpp:444-490)
     std::ofstream o = open_out(numbered(base, ".punish", num));
     o << "Limiting constraints (atomic density lies inside F boundaries)\n";
     o << std::right << setw(3) << "#" << setw(11) << "conf_w" << setw(15) << "punish error" << setw(15)

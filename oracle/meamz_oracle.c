@@ -16,7 +16,10 @@
  *     src/cell.cpp:158-170, Pair::setup_pair src/pair.cpp:35-42,
  *     check_pair src/pots/pot_meam_spline/pair_meam_spline.cpp:50-99,
  *     src/pots/pot_eam_spline/pair_eam_spline.cpp:50-86,
- *     src/pots/pot_meam/pair_meam.cpp:50-96, src/pots/pot_eam/pair_eam.cpp:50-85;
+ *     src/pots/pot_meam/pair_meam.cpp:50-96, src/pots/pot_eam/pair_eam.cpp:50-85,
+ *     src/pots/pot_pair_spline/pair_pair_spline.cpp:50-73,
+ *     src/pots/pot_pair/pair_pair.cpp:50-71 (pair styles: half list, j from i
+ *     on, src/config.cpp:179-184);
  *   - triplets (enumerated on the fly, same j<k order): find_triplets
  *     src/config.cpp:240-285, check_triplet
  *     src/pots/pot_meam_spline/triplet_meam_spline.cpp:49-73,
@@ -25,7 +28,9 @@
  *     src/pots/pot_meam_spline/pot_meam_spline.cpp:102-283,
  *     PotEAMSpline::fast_compute src/pots/pot_eam_spline/pot_eam_spline.cpp:100-224,
  *     PotMEAM::compute src/pots/pot_meam/pot_meam.cpp:95-275, PotEAM::compute
- *     src/pots/pot_eam/pot_eam.cpp:129-251, accumulate_error src/potential.cpp:415-491;
+ *     src/pots/pot_eam/pot_eam.cpp:129-251, PotPairSpline::fast_compute
+ *     src/pots/pot_pair_spline/pot_pair_spline.cpp:74-141, PotPair::compute
+ *     src/pots/pot_pair/pot_pair.cpp:73-144, accumulate_error src/potential.cpp:415-491;
  *   - alloy indices src/pot_fns.cpp:132-173.
  */
 #include "meamz_oracle.h"
@@ -323,8 +328,10 @@ double orc_splint_point(int basis, int n, double x0, double xn, const int* bc, c
 
 static int is_gmeam(const orc_ctx* c) { return c->style == ORC_GMEAM_SPLINE || c->style == ORC_GMEAM; }
 static int is_meam(const orc_ctx* c) { return c->style == ORC_MEAM_SPLINE || c->style == ORC_MEAM || is_gmeam(c); }
+static int is_pair(const orc_ctx* c) { return c->style == ORC_PAIR_SPLINE || c->style == ORC_PAIR; }
 static int is_spline_style(const orc_ctx* c) {
-  return c->style == ORC_EAM_SPLINE || c->style == ORC_MEAM_SPLINE || c->style == ORC_GMEAM_SPLINE;
+  return c->style == ORC_EAM_SPLINE || c->style == ORC_MEAM_SPLINE || c->style == ORC_GMEAM_SPLINE ||
+         c->style == ORC_PAIR_SPLINE;
 }
 
 /* PotFns::get_2body_alloy_idx Pair_ij, src/pot_fns.cpp:141-143 */
@@ -367,7 +374,7 @@ orc_ctx* orc_create(const orc_layout* lay, const orc_db* db, char* err) {
   c->embed_extrap = lay->embed_extrap;
   int nt = c->nt, npij = nt * (nt + 1) / 2;
   int counts[5] = {npij, nt, nt, npij, is_gmeam(c) ? nt * npij : nt};
-  int nfn = is_meam(c) ? 5 : 3;
+  int nfn = is_meam(c) ? 5 : (is_pair(c) ? 1 : 3);
   int tot = 0;
   for (int f = 0; f < nfn; ++f) { c->fn_off[f] = tot; tot += counts[f]; }
   if (tot != c->nspl) {
@@ -404,7 +411,7 @@ orc_ctx* orc_create(const orc_layout* lay, const orc_db* db, char* err) {
   /* max radial cutoff: phi, rho (, f) -- src/potential.cpp:73-82 */
   double max_rcut = 0.0;
   int radial[3] = {0, 1, 3};
-  for (int q = 0; q < (is_meam(c) ? 3 : 2); ++q) {
+  for (int q = 0; q < (is_meam(c) ? 3 : (is_pair(c) ? 1 : 2)); ++q) {
     int f = radial[q];
     for (int s = c->fn_off[f]; s < c->fn_off[f] + counts[f]; ++s)
       if (c->xn[s] > max_rcut) max_rcut = c->xn[s];
@@ -474,7 +481,8 @@ orc_ctx* orc_create(const orc_layout* lay, const orc_db* db, char* err) {
         c->cell_of[gi] = cc;
         if (pass == 0) c->poff[gi] = np;
         const double* pi = db->pos + 3 * gi;
-        for (int j = 0; j < nc; ++j) {
+        /* pair styles count each pair once: j starts at i (src/config.cpp:179-184) */
+        for (int j = is_pair(c) ? ia : 0; j < nc; ++j) {
           const double* pj = db->pos + 3 * (abase + j);
           for (int ix = -sx; ix <= sx; ++ix)
             for (int iy = -sy; iy <= sy; ++iy)
@@ -500,7 +508,7 @@ orc_ctx* orc_create(const orc_layout* lay, const orc_db* db, char* err) {
                 P.phi_knot = P.rho_knot = P.f_knot = -1;
                 P.phi_shift = P.rho_shift = P.f_shift = 0.0;
                 int bad = check_fn(c, &c->spl[P.phi_idx], P.r, &P.phi_knot, &P.phi_shift);
-                bad |= check_fn(c, &c->spl[P.rho_idx], P.r, &P.rho_knot, &P.rho_shift);
+                if (!is_pair(c)) bad |= check_fn(c, &c->spl[P.rho_idx], P.r, &P.rho_knot, &P.rho_shift);
                 if (is_meam(c)) bad |= check_fn(c, &c->spl[P.f_idx], P.r, &P.f_knot, &P.f_shift);
                 if (bad) {
                   if (err)
@@ -569,6 +577,7 @@ void orc_counts(const orc_ctx* c, long long* npairs, long long* ntriplets) {
 }
 
 int orc_errvec_size(const orc_ctx* c) {
+  if (is_pair(c)) return 7 * c->ncells + 3 * c->natoms;
   return 7 * c->ncells + 3 * c->natoms + c->ncells + (is_gmeam(c) ? c->nt * (c->nt + 1) / 2 : is_meam(c) ? 1 : c->nt);
 }
 
@@ -587,9 +596,40 @@ static double eval_fn(const orc_ctx* c, const spl_t* s, int knot, double shift, 
   return spl_comb_point(s, r, grad, NULL);
 }
 
+/* Pair styles: one pass over the half list; a pair whose neighbour is the atom
+ * itself (a periodic image) counts half, src/pots/pot_pair_spline/
+ * pot_pair_spline.cpp:98-134 (knot form), src/pots/pot_pair/pot_pair.cpp:99-135
+ * (point form). */
+static void pair_body(const orc_ctx* c, int a, accum_t* acc) {
+  const int cc = c->cell_of[a];
+  for (long long p = c->poff[a]; p < c->poff[a + 1]; ++p) {
+    const pair_t* P = &c->pairs[p];
+    if (P->phi_knot == -1) continue;
+    double phigrad;
+    double phival = eval_fn(c, &c->spl[P->phi_idx], P->phi_knot, P->phi_shift, P->r, &phigrad);
+    if (P->j == a) {
+      phival *= 0.5;
+      phigrad *= 0.5;
+    }
+    acc->energy[cc] += phival;
+    double t[3] = {P->dist[0] * phigrad, P->dist[1] * phigrad, P->dist[2] * phigrad};
+    double* fi = acc->force + 3 * a; double* fj = acc->force + 3 * P->j;
+    fi[0] += t[0]; fi[1] += t[1]; fi[2] += t[2];
+    fj[0] -= t[0]; fj[1] -= t[1]; fj[2] -= t[2];
+    t[0] *= P->r; t[1] *= P->r; t[2] *= P->r;
+    double* s = acc->stress + 6 * cc;
+    s[0] -= P->dist[0] * t[0]; s[1] -= P->dist[1] * t[1]; s[2] -= P->dist[2] * t[2];
+    s[3] -= P->dist[0] * t[1]; s[4] -= P->dist[1] * t[2]; s[5] -= P->dist[2] * t[0];
+  }
+}
+
 /* The per-atom body of fast_compute / compute (pair loop, triplet loop,
  * embedding, pair force loop, triplet force loop). */
 static void atom_body(const orc_ctx* c, int a, accum_t* acc) {
+  if (is_pair(c)) {
+    pair_body(c, a, acc);
+    return;
+  }
   const int meam = is_meam(c);
   const int cc = c->cell_of[a];
   const double w = c->weight[cc];
@@ -750,6 +790,7 @@ static double finish(const orc_ctx* c, const accum_t* acc, double* errvec, doubl
         if (forces) forces[3 * a + m] = fv;
       }
   }
+  if (is_pair(c)) return F; /* no constraint rows, no penalty (extra_error is empty) */
   for (int cc = 0; cc < c->ncells; ++cc) {
     double v = acc->constraint[cc];
     F += v * v;

@@ -14,11 +14,21 @@
 extern "C" {
 #endif
 
-enum { ORC_EAM_SPLINE = 0, ORC_MEAM_SPLINE = 1, ORC_EAM = 2, ORC_MEAM = 3, ORC_GMEAM_SPLINE = 4, ORC_GMEAM = 5 };
+enum {
+  ORC_EAM_SPLINE = 0,
+  ORC_MEAM_SPLINE = 1,
+  ORC_EAM = 2,
+  ORC_MEAM = 3,
+  ORC_GMEAM_SPLINE = 4,
+  ORC_GMEAM = 5,
+  ORC_PAIR_SPLINE = 6,
+  ORC_PAIR = 7
+};
 
 /* One potential's spline layout.  Splines are listed in file order
  * (phi Pair_ij, rho Pair_j, U Pair_i [, f Pair_ij, g Triplet_i; gmeam styles:
- * g Triplet_i_jk, ntypes * ntypes (ntypes + 1) / 2 splines]).
+ * g Triplet_i_jk, ntypes * ntypes (ntypes + 1) / 2 splines]; pair styles: phi
+ * Pair_ij only).
  * bc holds two ints per spline: -1 natural, 0 free slope, 1 fixed slope. */
 typedef struct {
   int style;

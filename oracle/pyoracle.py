@@ -14,7 +14,8 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libmeamz_oracle.so")
-STYLE_CODE = {"eam/spline": 0, "meam/spline": 1, "eam": 2, "meam": 3, "gmeam/spline": 4, "gmeam": 5}
+STYLE_CODE = {"eam/spline": 0, "meam/spline": 1, "eam": 2, "meam": 3, "gmeam/spline": 4, "gmeam": 5,
+              "pair/spline": 6, "pair": 7}
 
 
 def build() -> str:

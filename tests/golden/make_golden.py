@@ -188,6 +188,7 @@ def main():
             note="5 atoms: bonds at exactly the f cutoff, collinear triplet, g on [-0.5,0.5]")
 
     gmeam_cases()
+    pair_cases()
 
 
 def gmeam_cases():
@@ -210,8 +211,37 @@ def gmeam_cases():
             note="binary gmeam (point style), f_AB on [1.8,4.2]")
 
 
+def pair_cases():
+    """Pair styles: phi alone, each pair once (half list), a pair with the
+    atom's own periodic image counting half, no density or penalty rows.
+    ``python tests/golden/make_golden.py pair`` writes only these cases."""
+    rng = np.random.Generator(np.random.PCG64(6161))
+    w = gen.workload(2, scale=8 / 512)
+    small = [c for c in range(w.db.ncells) if w.db.natoms[c] <= 200][:6]
+    nk = dict(phi=10)
+    pot = gen.make_potential(rng, "pair/spline", 2, 5.0, nk, (-1.0, 3.0))
+    truth = gen.make_potential(rng, "pair/spline", 2, 5.0, nk, (-1.0, 3.0))
+    _finish("pair_spline_binary", pot, _slice(w, small[:4]), truth=truth, fd=1e-6,
+            note="binary pair/spline (3 phi splines), FD gradient")
+    pot = pot.copy()
+    pot.style = "pair"
+    pot.splines[1].xn = 4.2          # phi_AB ends before the other cutoffs
+    _finish("pair_point_binary", pot, _slice(w, small[:3]), truth=truth,
+            note="binary pair (point style), phi_AB on [1.8,4.2]")
+    # one atom per cell: every pair is a periodic image of the atom itself
+    w1 = gen.workload(0, scale=1 / 16)
+    db = w1.db.subset([0])
+    one = gen._empty_db([(np.zeros((1, 3)), db.box[0] * 0.5, np.zeros(1, int))])
+    pot1 = gen.make_potential(rng, "pair/spline", 1, 5.0, nk, (-1.0, 3.0))
+    truth1 = gen.make_potential(rng, "pair/spline", 1, 5.0, nk, (-1.0, 3.0))
+    _finish("pair_spline_self", pot1, one, truth=truth1, fd=1e-6,
+            note="one atom per cell: only self-image pairs (half weight)")
+
+
 if __name__ == "__main__":
     if len(sys.argv) > 1 and sys.argv[1] == "gmeam":
         gmeam_cases()
+    elif len(sys.argv) > 1 and sys.argv[1] == "pair":
+        pair_cases()
     else:
         main()

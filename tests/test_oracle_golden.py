@@ -6,6 +6,7 @@ import pytest
 from conftest import golden_cases, load_case, rel_err
 
 import pyoracle
+from meamzilla_amd import formats
 
 
 @pytest.mark.parametrize("case", golden_cases())
@@ -18,7 +19,10 @@ def test_oracle_matches_reference(case):
     assert rel_err(r["cell_energy"], exp["cell_energy"]) < 1e-12
     assert rel_err(r["cell_stress"], exp["cell_stress"]) < 1e-12
     assert rel_err(r["forces"], exp["forces"]) < 1e-12
-    assert rel_err(o.densities(), exp["densities"]) < 1e-12
+    if formats.is_pair_style(pot.style):
+        assert exp["densities"] == []  # the reference computes none for pair styles
+    else:
+        assert rel_err(o.densities(), exp["densities"]) < 1e-12
 
 
 @pytest.mark.parametrize("case", golden_cases())
