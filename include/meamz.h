@@ -12,9 +12,10 @@
  * thread per context.  All doubles are IEEE binary64.
  *
  * Knot vector.  A potential is a list of splines in file order (phi Pair_ij,
- * rho Pair_j, U Pair_i and, for MEAM, f Pair_ij and g Triplet_i; pair styles
- * hold phi Pair_ij alone and their residual vector has no density or penalty
- * rows).  Each spline
+ * rho Pair_j, U Pair_i and, for MEAM, f Pair_ij and g Triplet_i; jmeam styles
+ * then p Pair_ij, q Triplet_i (and, for jmeam2, p2 Pair_ij, q2 Triplet_i); pair
+ * styles hold phi Pair_ij alone and their residual vector has no density or
+ * penalty rows).  Each spline
  * contributes n+4 doubles: its n knot values y_0..y_{n-1}, then grad_L, grad_R,
  * grad2_L, grad2_R (boundary slopes and, for quintic splines, boundary second
  * derivatives).  Every evaluation entry point takes such a vector.
@@ -43,6 +44,10 @@ extern "C" {
 #define MZ_STYLE_GMEAM 5       /* "gmeam" (point-evaluated) */
 #define MZ_STYLE_PAIR_SPLINE 6 /* "pair/spline": phi only (src/pots/pot_pair_spline/pot_pair_spline.cpp:74-141) */
 #define MZ_STYLE_PAIR 7        /* "pair" (point-evaluated, src/pots/pot_pair/pot_pair.cpp:73-144) */
+#define MZ_STYLE_JMEAM_SPLINE 8  /* "jmeam/spline": meam plus p_ij p_ik q(cos) energy terms */
+#define MZ_STYLE_JMEAM 9         /* "jmeam" (point-evaluated, src/pots/pot_jmeam/pot_jmeam.cpp:101-300) */
+#define MZ_STYLE_JMEAM2_SPLINE 10 /* "jmeam2/spline": two p p q terms (p, q and p2, q2) */
+#define MZ_STYLE_JMEAM2 11       /* "jmeam2" (point-evaluated, src/pots/pot_jmeam2/pot_jmeam2.cpp:107-330) */
 
 /* Spline layout of one potential.  bc holds two ints per spline:
  * -1 natural, 0 free slope, 1 fixed slope (the "#B" constraint line). */
@@ -122,14 +127,17 @@ int mz_densities(mz_ctx* ctx, const double* knots, double* dens);
 
 /* Objective and its gradient with respect to every entry of the knot vector
  * (dF has info.nknot entries; entries of fixed knots and natural boundary
- * slots are still filled).  One reverse sweep over the same lists.
+ * slots are still filled).  One reverse sweep over the same lists.  The
+ * jmeam styles have no adjoint: MZ_ERR_LAYOUT (use mz_objective_batch
+ * without G, or mz_residuals_batch, for their difference quotients).
  * No reference counterpart: the reference's Powell forms forward differences,
  * src/opt_powell.cpp:348-389. */
 int mz_objective_grad(mz_ctx* ctx, const double* knots, double* F, double* dF);
 
 /* Objective (and, if G is non-NULL, gradient) of ncand knot vectors stored
  * row-major in knots[ncand*nknot], all scored over the whole database in one
- * captured launch sequence.  F[ncand], G[ncand*nknot].
+ * captured launch sequence.  F[ncand], G[ncand*nknot] (G must be NULL for
+ * the jmeam styles).
  * No reference counterpart: replaces the per-member compute loops of the GA
  * src/opt_genalg.cpp:143-150,237-265 and the trial loops of simulated
  * annealing src/opt_simann.cpp:120-135 and the line searches. */

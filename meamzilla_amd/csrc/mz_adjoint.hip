@@ -970,6 +970,7 @@ static int run_grad(mz_ctx* c, int ncand) {
 
 int mz_grad_impl(mz_ctx* c, int ncand, const double* knots, double* F, double* G) {
   if (!c || !knots || !F || !G || ncand < 1) return mz_fail(MZ_ERR_ARG, "null argument or ncand < 1");
+  if (c->pot.njm) return mz_fail(MZ_ERR_LAYOUT, "no adjoint gradient for the jmeam styles (objective and residuals only)");
   int rc = setup_adjoint(c);
   if (rc) return rc;
   mz::AdjointExtra& X = c->adjx;

@@ -19,7 +19,11 @@ enum Style : int {
   kGMeamSpline = 4,
   kGMeam = 5,
   kPairSpline = 6,
-  kPair = 7
+  kPair = 7,
+  kJMeamSpline = 8,
+  kJMeam = 9,
+  kJMeam2Spline = 10,
+  kJMeam2 = 11
 };
 
 // Per-spline header in the device table (16 doubles = 128 bytes).
@@ -63,7 +67,7 @@ struct PotDesc {
   int32_t nt;
   int32_t nspl;
   int32_t embed_extrap;
-  int32_t fn_off[5];    // first spline of phi, rho, U, f, g
+  int32_t fn_off[9];    // first spline of phi, rho, U, f, g, p, q, p2, q2
   int32_t ncoef;        // total interval records in one table
   int32_t table_bytes;  // heads + intervals of one table
   int32_t nknot;        // knot-vector length
@@ -74,6 +78,7 @@ struct PotDesc {
   int32_t gpair;        // g indexed by (type i, types j k) (gmeam styles, more than one type)
   int32_t ng;           // number of g splines (nt, or nt * nt (nt + 1) / 2 with gpair)
   int32_t nodens;       // pair styles: phi only, no density, embedding or density penalty
+  int32_t njm;          // jmeam styles: extra p_ij p_ik q(cos) energy terms (1, or 2 for jmeam2)
 };
 
 // Doubles per g-interval record in the atom kernel's compact LDS copy

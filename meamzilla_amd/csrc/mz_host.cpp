@@ -45,21 +45,27 @@ int pair_ij_h(int ti, int tj, int nt) {
 namespace {
 
 int build_layout(mz_ctx* c, const mz_layout* L) {
-  if (!L || L->style < 0 || L->style > 7) return fail(MZ_ERR_LAYOUT, "unknown style");
+  if (!L || L->style < 0 || L->style > 11) return fail(MZ_ERR_LAYOUT, "unknown style");
   if (L->ntypes < 1 || L->ntypes > mz::kMaxTypes) return fail(MZ_ERR_LAYOUT, "ntypes out of range");
   c->style = L->style;
   c->nt = L->ntypes;
   const bool gmeam = L->style == mz::kGMeamSpline || L->style == mz::kGMeam;
-  c->meam = (L->style == mz::kMeamSpline || L->style == mz::kMeam || gmeam);
-  c->point = (L->style == mz::kEam || L->style == mz::kMeam || L->style == mz::kGMeam || L->style == mz::kPair);
+  // jmeam styles: MEAM plus p Pair_ij, q Triplet_i (jmeam2: also p2, q2),
+  // src/pots/pot_jmeam/pot_jmeam.cpp:30-47, src/pots/pot_jmeam2/pot_jmeam2.cpp:30-46
+  const int njm = (L->style == mz::kJMeamSpline || L->style == mz::kJMeam)     ? 1
+                  : (L->style == mz::kJMeam2Spline || L->style == mz::kJMeam2) ? 2
+                                                                                 : 0;
+  c->meam = (L->style == mz::kMeamSpline || L->style == mz::kMeam || gmeam || njm > 0);
+  c->point = (L->style == mz::kEam || L->style == mz::kMeam || L->style == mz::kGMeam || L->style == mz::kPair ||
+              L->style == mz::kJMeam || L->style == mz::kJMeam2);
   // pair styles: phi Pair_ij alone (src/pots/pot_pair/pot_pair.cpp:24-37)
   c->pair = (L->style == mz::kPairSpline || L->style == mz::kPair);
   const int nt = c->nt, npij = nt * (nt + 1) / 2;
   // g: Triplet_i (meam), Triplet_i_jk (gmeam, src/pots/pot_gmeam/pot_gmeam.cpp:31-36)
-  const int counts[5] = {npij, nt, nt, npij, gmeam ? nt * npij : nt};
-  const int nfn = c->meam ? 5 : (c->pair ? 1 : 3);
+  const int counts[9] = {npij, nt, nt, npij, gmeam ? nt * npij : nt, npij, nt, npij, nt};
+  const int nfn = c->meam ? 5 + 2 * njm : (c->pair ? 1 : 3);
   int tot = 0;
-  for (int f = 0; f < 5; ++f) c->pot.fn_off[f] = 0;
+  for (int f = 0; f < 9; ++f) c->pot.fn_off[f] = 0;
   for (int f = 0; f < nfn; ++f) {
     c->pot.fn_off[f] = tot;
     tot += counts[f];
@@ -108,7 +114,8 @@ int build_layout(mz_ctx* c, const mz_layout* L) {
   c->pot.ng = c->meam ? counts[4] : 0;
   c->pot.gpair = gmeam && nt > 1;  // one type: gmeam is meam
   c->pot.nodens = c->pair ? 1 : 0;
-  c->nfpen = gmeam ? npij : (c->meam ? 1 : 0);
+  c->pot.njm = njm;
+  c->nfpen = gmeam ? npij : (c->meam ? 1 + njm : 0);  // f-max, then one p-max per p function
   // |cos| <= 1 + a few ulps, so a knot-style g spline ending at or beyond 1
   // never cuts a triplet
   c->pot.g_keep_all = 0;
@@ -196,9 +203,10 @@ void cell_pairs(const mz_ctx* c, int cc, const double* box, const double* pos, d
             const double r = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
             if (r >= max_rcut) continue;
             // check_pair: first-knot test and inside flags
-            const int fns[3] = {c->pot.fn_off[0] + pij, c->pot.fn_off[1] + tj, c->pot.fn_off[3] + pij};
+            const int fns[5] = {c->pot.fn_off[0] + pij, c->pot.fn_off[1] + tj, c->pot.fn_off[3] + pij,
+                                c->pot.fn_off[5] + pij, c->pot.fn_off[7] + pij};
             bool inside = false;
-            for (int q = 0; q < (c->meam ? 3 : (c->pair ? 1 : 2)); ++q) {
+            for (int q = 0; q < (c->meam ? 3 + c->pot.njm : (c->pair ? 1 : 2)); ++q) {
               const Grid& g = c->grids[fns[q]];
               if (r < g.x[0]) {
                 char buf[200];
@@ -216,9 +224,52 @@ void cell_pairs(const mz_ctx* c, int cc, const double* box, const double* pos, d
   }
 }
 
+// jmeam styles: a triplet is kept when any of its terms is present -- the
+// cosine inside g or q (spline styles), both legs inside f or p (point
+// styles); check_triplet src/pots/pot_jmeam/triplet_jmeam.cpp:48-88,
+// src/pots/pot_jmeam_spline/triplet_jmeam_spline.cpp:49-88.
+long long count_triplets_jmeam(const mz_ctx* c, int gi, const double* pos, const double* box) {
+  const long long b = c->atom_poff[gi], e = c->atom_poff[gi + 1], n = e - b;
+  const int ti = c->types[gi], nt = c->nt, nterm = 1 + c->pot.njm;
+  std::vector<double> u(3 * (size_t)n), rr((size_t)n);
+  std::vector<int> tj((size_t)n);
+  for (long long p = b; p < e; ++p) {
+    const int j = (int)c->pairs[p].x;
+    const uint32_t im = c->pairs[p].y;
+    const int ix = (int8_t)(im & 0xff), iy = (int8_t)((im >> 8) & 0xff), iz = (int8_t)((im >> 16) & 0xff);
+    const double* pj = pos + 3 * (size_t)j;
+    const double* pi = pos + 3 * (size_t)gi;
+    double d[3];
+    for (int q = 0; q < 3; ++q) d[q] = pj[q] + ix * box[q] + iy * box[3 + q] + iz * box[6 + q] - pi[q];
+    const double r = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    for (int q = 0; q < 3; ++q) u[3 * (p - b) + q] = d[q] / r;
+    rr[p - b] = r;
+    tj[p - b] = c->types[j];
+  }
+  long long cnt = 0;
+  for (long long p = 0; p < n - 1; ++p)
+    for (long long q = p + 1; q < n; ++q) {
+      bool keep = false;
+      for (int m = 0; m < nterm && !keep; ++m) {
+        if (c->point) {
+          const int f = m == 0 ? 3 : 3 + 2 * m;
+          const Grid& gj = c->grids[c->pot.fn_off[f] + pair_ij_h(ti, tj[p], nt)];
+          const Grid& gk = c->grids[c->pot.fn_off[f] + pair_ij_h(ti, tj[q], nt)];
+          keep = rr[p] <= gj.x[gj.n - 1] && rr[q] <= gk.x[gk.n - 1];
+        } else {
+          const double cs = u[3 * p] * u[3 * q] + u[3 * p + 1] * u[3 * q + 1] + u[3 * p + 2] * u[3 * q + 2];
+          keep = cs < c->grids[c->pot.fn_off[4 + 2 * m] + ti].xcut();
+        }
+      }
+      cnt += keep;
+    }
+  return cnt;
+}
+
 // Triplet count of one atom as the reference's find_triplets keeps them.
 long long count_triplets(const mz_ctx* c, int gi, const double* pos, const double* box) {
   if (!c->meam) return 0;
+  if (c->pot.njm) return count_triplets_jmeam(c, gi, pos, box);
   const long long b = c->atom_poff[gi], e = c->atom_poff[gi + 1];
   const int ti = c->types[gi];
   const int nt = c->nt;
@@ -440,6 +491,8 @@ mz::FinalArgs final_args(const mz_ctx* c) {
   f.fper = c->meam && c->nfpen == f.nf && (c->style == mz::kGMeamSpline || c->style == mz::kGMeam);
   f.u0 = c->pot.fn_off[2];
   f.nodens = c->pot.nodens;
+  f.npgrp = c->meam && !f.fper ? 1 + c->pot.njm : 0;
+  for (int g = 0; g < 3; ++g) f.pgrp0[g] = c->pot.fn_off[g == 0 ? 3 : 3 + 2 * g];
   f.koff = c->d_koff.p;
   f.nknots = c->d_nknots.p;
   f.knots = c->d_knots.p;
@@ -469,7 +522,8 @@ int mz_run_forward(mz_ctx* c, int ncand, double* d_errvec, double* d_ce, double*
   // candidates per workgroup: the tile's choice, at most what ncand fills and
   // what the LDS holds
   int ct = std::min(c->tile.cand_tile, ncand >= 4 ? 4 : (ncand >= 2 ? 2 : 1));
-  while (ct > 1 && mz::atoms_static_lds(c->tile.block, ppl, ct, c->pot.gpair != 0) + mz::atoms_dyn_lds(c->pot, ct) > 64 * 1024) ct >>= 1;
+  if (c->pot.njm) ct = 1;  // the jmeam kernels take one candidate per workgroup
+  while (ct > 1 && mz::atoms_static_lds(c->tile.block, ppl, ct, c->pot) + mz::atoms_dyn_lds(c->pot, ct) > 64 * 1024) ct >>= 1;
   if (mz::launch_atoms(st, aa, (int)c->chunks.size(), ncand, c->tile.block, ppl, ct))
     return fail(MZ_ERR_ARG, "unsupported tile (block, pairs per lane)");
   MZ_HIP(hipGetLastError());
@@ -604,10 +658,10 @@ int mz_create(const mz_layout* layout, const mz_db* db, mz_ctx** out) {
   }
   // max radial cutoff (src/potential.cpp:73-82)
   double max_rcut = 0.0;
-  const int radial[3] = {0, 1, 3};
+  const int radial[5] = {0, 1, 3, 5, 7};
   const int nt = c->nt, npij = nt * (nt + 1) / 2;
-  const int counts[5] = {npij, nt, nt, npij, nt};
-  for (int q = 0; q < (c->meam ? 3 : (c->pair ? 1 : 2)); ++q)
+  const int counts[9] = {npij, nt, nt, npij, nt, npij, nt, npij, nt};
+  for (int q = 0; q < (c->meam ? 3 + c->pot.njm : (c->pair ? 1 : 2)); ++q)
     for (int s = c->pot.fn_off[radial[q]]; s < c->pot.fn_off[radial[q]] + counts[radial[q]]; ++s)
       max_rcut = std::max(max_rcut, c->xn[s]);
 
@@ -813,7 +867,7 @@ int mz_set_tile(mz_ctx* c, const mz_tile* t) {
   if (!ok || ppl * t->block != t->max_pairs) return fail(MZ_ERR_ARG, "unsupported tile (block, max_pairs)");
   if (t->cand_tile != 0 && t->cand_tile != 1 && t->cand_tile != 2 && t->cand_tile != 4)
     return fail(MZ_ERR_ARG, "cand_tile must be 1, 2 or 4");
-  if (mz::atoms_static_lds(t->block, ppl, 1, c->pot.gpair != 0) + mz::atoms_dyn_lds(c->pot, 1) > 64 * 1024)
+  if (mz::atoms_static_lds(t->block, ppl, 1, c->pot) + mz::atoms_dyn_lds(c->pot, 1) > 64 * 1024)
     return fail(MZ_ERR_CAPACITY, "tile and spline tables exceed 64 KB of LDS");
   mz_tile old = c->tile;
   drop_graphs(c);

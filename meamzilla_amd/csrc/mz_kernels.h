@@ -84,6 +84,8 @@ struct FinalArgs {
   int f0, nf, u0;
   int fper;               // one f-max penalty per f spline (gmeam) instead of one over all
   int nodens;             // pair styles: no penalty term
+  int npgrp;              // max-|y| penalty groups: f, then (jmeam) p and p2
+  int pgrp0[3];           // first spline of each group (nf splines each)
   const int* koff;
   const int* nknots;
   const double* knots;
@@ -101,7 +103,7 @@ int launch_atoms(hipStream_t st, const AtomArgs& a, int nchunks, int ncand, int 
 // Dynamic (spline tables) and static LDS bytes of one atom workgroup.
 bool atoms_tables_global(const PotDesc& pot, int ct);  // k_atoms reads spline tables from global memory
 size_t atoms_dyn_lds(const PotDesc& pot, int ct);
-size_t atoms_static_lds(int block, int ppl, int ct, bool gpair);
+size_t atoms_static_lds(int block, int ppl, int ct, const PotDesc& pot);
 int resid_blocks(int natoms, int ncells);
 void launch_resid(hipStream_t st, const ResidArgs& a, int nblocks, int ncand);
 void launch_final(hipStream_t st, const FinalArgs& a, int ncand);

@@ -314,7 +314,7 @@ __device__ __forceinline__ void g_value(const SplHead& Hg, const double* gtg, co
 
 // LDS layout of one workgroup (static part).  CT candidates share the pair
 // geometry; f of candidate 0 rides in the partner record, the others in fx.
-template <int BLOCK, int PPL, int MAXA, int CT, bool GP>
+template <int BLOCK, int PPL, int MAXA, int CT, bool GP, int JM>
 struct AtomSmem {
   double4 rec[BLOCK * PPL];                 // per pair: unit vector, f (candidate 0); later T
   double fx[CT > 1 ? CT - 1 : 1][BLOCK * PPL];  // f of candidates 1..CT-1
@@ -324,16 +324,21 @@ struct AtomSmem {
   int poff[MAXA + 1];
   double red8[CT][8][BLOCK / 64];
   int ptype[GP ? BLOCK * PPL : 1];          // per pair: type of its neighbour (g by pair types)
+  double px[JM > 0 ? JM * CT : 1][JM > 0 ? BLOCK * PPL : 1];  // jmeam: p (p2) of every pair, per candidate
 };
 
 // One workgroup per (chunk, group of CT candidates).  TG: the spline tables
 // are read from global memory instead of being staged in LDS (large tables).
 // GP: the g spline of a triplet depends on the types of both neighbours
 // (gmeam styles, src/pot_fns.cpp:154-171 Triplet_i_jk), chosen per triplet.
-template <int BLOCK, int PPL, int MAXA, int CT, bool MEAM, bool POINT, bool Q, bool TG, bool GP>
+// JM: jmeam styles, JM extra energy terms p_ij p_ik q(cos) per triplet
+// (src/pots/pot_jmeam/pot_jmeam.cpp:221,263-265; jmeam2 adds p2 p2 q2,
+// src/pots/pot_jmeam2/pot_jmeam2.cpp:240-241,284-286), walked beside g.
+template <int BLOCK, int PPL, int MAXA, int CT, bool MEAM, bool POINT, bool Q, bool TG, bool GP, int JM>
 __global__ void __launch_bounds__(BLOCK) k_atoms(const AtomArgs args) {
   extern __shared__ __attribute__((aligned(64))) char dyn[];
-  __shared__ __attribute__((aligned(16))) AtomSmem<BLOCK, PPL, MAXA, CT, GP> sm;
+  __shared__ __attribute__((aligned(16))) AtomSmem<BLOCK, PPL, MAXA, CT, GP, JM> sm;
+  constexpr int NJ = JM > 0 ? JM : 1;
   constexpr int NG = g_record_doubles(Q);
   const Chunk ch = args.chunks[blockIdx.x];
   const int tid = threadIdx.x;
@@ -395,6 +400,8 @@ __global__ void __launch_bounds__(BLOCK) k_atoms(const AtomArgs args) {
   double r[PPL], invr[PPL], ux[PPL], uy[PPL], uz[PPL];
   double aphi[PPL][CT], brho[PPL][CT], fv[PPL][CT], dfv[PPL][CT], Gx[PPL][CT], Gy[PPL][CT], Gz[PPL][CT],
       S1v[PPL][CT];
+  // jmeam: p and p' of the lane's pairs, and the legs H of the p p q terms
+  double pv[NJ][PPL][CT], dpv[NJ][PPL][CT], Hx[PPL][CT], Hy[PPL][CT], Hz[PPL][CT];
   double e_acc[CT];
 #pragma unroll
   for (int c = 0; c < CT; ++c) e_acc[c] = 0.0;
@@ -408,6 +415,9 @@ __global__ void __launch_bounds__(BLOCK) k_atoms(const AtomArgs args) {
     for (int c = 0; c < CT; ++c) {
       aphi[s][c] = brho[s][c] = fv[s][c] = dfv[s][c] = 0.0;
       Gx[s][c] = Gy[s][c] = Gz[s][c] = S1v[s][c] = 0.0;
+      Hx[s][c] = Hy[s][c] = Hz[s][c] = 0.0;
+#pragma unroll
+      for (int m = 0; m < NJ; ++m) pv[m][s][c] = dpv[m][s][c] = 0.0;
     }
     if (p < ch.np) {
       int lo = 0, hi = ch.na - 1;
@@ -473,6 +483,16 @@ __global__ void __launch_bounds__(BLOCK) k_atoms(const AtomArgs args) {
         sm.rec[p] = make_double4(ux[s], uy[s], uz[s], fv[s][0]);
 #pragma unroll
         for (int c = 1; c < CT; ++c) sm.fx[c - 1][p] = fv[s][c];
+#pragma unroll
+        for (int m = 0; m < JM; ++m) {
+          const int sp = args.pot.fn_off[5 + 2 * m] + pij;
+          if (locate_radial<POINT>(HD0[sp], IV0, rr, k, t, reg)) {
+#pragma unroll
+            for (int c = 0; c < CT; ++c) eval_located<Q>(HDc(c)[sp], IVc(c), k, t, reg, pv[m][s][c], dpv[m][s][c]);
+          }
+#pragma unroll
+          for (int c = 0; c < CT; ++c) sm.px[m * CT + c][p] = pv[m][s][c];
+        }
       }
     }
   }
@@ -498,11 +518,15 @@ __global__ void __launch_bounds__(BLOCK) k_atoms(const AtomArgs args) {
       const SplHead& Hg0 = GHc(0)[gfix];
       const int goff0 = (Hg0.coef_off - args.pot.gco0) * NG;
       const double xj = ux[s], yj = uy[s], zj = uz[s];
-      double S[CT][5];
+      double S[CT][5], T[NJ][CT][5];
 #pragma unroll
       for (int c = 0; c < CT; ++c)
 #pragma unroll
-        for (int q = 0; q < 5; ++q) S[c][q] = 0.0;
+        for (int q = 0; q < 5; ++q) {
+          S[c][q] = 0.0;
+#pragma unroll
+          for (int m = 0; m < NJ; ++m) T[m][c][q] = 0.0;
+        }
       for (int k = kb; k < ke; ++k) {
         const double4 P = sm.rec[k];
         const double cs = xj * P.x + yj * P.y + zj * P.z;
@@ -523,6 +547,28 @@ __global__ void __launch_bounds__(BLOCK) k_atoms(const AtomArgs args) {
           S[c][3] += b * P.y;
           S[c][4] += b * P.z;
         }
+        // jmeam: q (q2) of the atom's type at the same cosine; the term is
+        // absent for a knot-style cosine beyond the spline
+#pragma unroll
+        for (int m = 0; m < JM; ++m) {
+          const int iq = args.pot.fn_off[6 + 2 * m] + ti;
+          const SplHead& Hq = HDc(0)[iq];
+          const GLoc Lq = g_locate<POINT, Q>(Hq, cs);
+          const bool zq = k == p || (!POINT && !(cs < Hq.xcut));
+          const int qoff = (Hq.coef_off - args.pot.gco0) * NG;
+#pragma unroll
+          for (int c = 0; c < CT; ++c) {
+            double qv, qp;
+            g_value<POINT, Q>(HDc(c)[iq], GTc(c) + qoff, Lq, qv, qp);
+            const double pk = zq ? 0.0 : sm.px[m * CT + c][k];
+            const double b = qp * pk;
+            T[m][c][0] += qv * pk;
+            T[m][c][1] += b * cs;
+            T[m][c][2] += b * P.x;
+            T[m][c][3] += b * P.y;
+            T[m][c][4] += b * P.z;
+          }
+        }
       }
 #pragma unroll
       for (int c = 0; c < CT; ++c) {
@@ -533,6 +579,17 @@ __global__ void __launch_bounds__(BLOCK) k_atoms(const AtomArgs args) {
         Gy[s][c] = yj * radial + fr * S[c][3];
         Gz[s][c] = zj * radial + fr * S[c][4];
         S1v[s][c] = S[c][0];
+        // p p q terms enter the energy directly (each unordered triplet is
+        // seen from both legs) and their legs bypass U'
+#pragma unroll
+        for (int m = 0; m < JM; ++m) {
+          e_acc[c] += 0.5 * pv[m][s][c] * T[m][c][0];
+          const double pr = pv[m][s][c] * invr[s];
+          const double rq = dpv[m][s][c] * T[m][c][0] - pr * T[m][c][1];
+          Hx[s][c] += xj * rq + pr * T[m][c][2];
+          Hy[s][c] += yj * rq + pr * T[m][c][3];
+          Hz[s][c] += zj * rq + pr * T[m][c][4];
+        }
       }
     }
   }
@@ -600,9 +657,14 @@ __global__ void __launch_bounds__(BLOCK) k_atoms(const AtomArgs args) {
       const int p = tid + s * BLOCK;
       const double dFi = sm.dF[c][ia[s]];
       const double sc = aphi[s][c] + brho[s][c] * dFi;
-      const double tx = ux[s] * sc + dFi * Gx[s][c];
-      const double ty = uy[s] * sc + dFi * Gy[s][c];
-      const double tz = uz[s] * sc + dFi * Gz[s][c];
+      double tx = ux[s] * sc + dFi * Gx[s][c];
+      double ty = uy[s] * sc + dFi * Gy[s][c];
+      double tz = uz[s] * sc + dFi * Gz[s][c];
+      if (JM) {
+        tx += Hx[s][c];
+        ty += Hy[s][c];
+        tz += Hz[s][c];
+      }
       const size_t pg = (size_t)cand[c] * args.npairs + ch.p0 + p;
       if (valid[c]) {
         if (args.G_out)
@@ -760,20 +822,24 @@ __global__ void __launch_bounds__(256) k_final(const FinalArgs a) {
       if (ev) ev[tail + q] = pe;
     }
   } else if (a.meam) {
-    // PotFns::get_max_y_mag over the f splines (src/pot_fns.cpp:100-111)
-    double mx = 0.0;
-    for (int q = 0; q < a.nf; ++q) {
-      const int s2 = a.f0 + q;
-      const double* y = kv + a.koff[s2];
-      double m = 0.0;
-      for (int k = 0; k < a.nknots[s2]; ++k)
-        if (fabs(m) < fabs(y[k])) m = y[k];
-      if (fabs(mx) < fabs(m)) mx = m;
+    // PotFns::get_max_y_mag over the f splines (src/pot_fns.cpp:100-111); the
+    // jmeam styles add one such term over the p (and p2) splines
+    // (src/pots/pot_jmeam2/pot_jmeam2.cpp:308-324)
+    for (int g = 0; g < a.npgrp; ++g) {
+      double mx = 0.0;
+      for (int q = 0; q < a.nf; ++q) {
+        const int s2 = a.pgrp0[g] + q;
+        const double* y = kv + a.koff[s2];
+        double m = 0.0;
+        for (int k = 0; k < a.nknots[s2]; ++k)
+          if (fabs(m) < fabs(y[k])) m = y[k];
+        if (fabs(mx) < fabs(m)) mx = m;
+      }
+      const double m = fabs(mx);
+      const double pe = kDummyWeight * 25. * (1.0 - m) * (1.0 - m);
+      F += pe * pe;
+      if (ev) ev[tail + g] = pe;
     }
-    const double m = fabs(mx);
-    const double pe = kDummyWeight * 25. * (1.0 - m) * (1.0 - m);
-    F += pe * pe;
-    if (ev) ev[tail] = pe;
   } else {
     const char* tb = a.tables + (size_t)cand * a.table_bytes;
     const SplHead* HD = reinterpret_cast<const SplHead*>(tb);
@@ -809,13 +875,13 @@ template <int BLOCK, int PPL, int MAXA, int CT>
 static void launch_atoms_t(hipStream_t st, const AtomArgs& a, int nchunks, int ncand) {
   const int style = a.pot.style;
   const bool meam = style == kMeamSpline || style == kMeam || style == kGMeamSpline || style == kGMeam;
-  const bool point = style == kEam || style == kMeam || style == kGMeam;
+  const bool point = style == kEam || style == kMeam || style == kGMeam || style == kPair;
   const bool q = a.pot.quintic != 0;
   const size_t dyn = atoms_dyn_lds(a.pot, CT);
   const bool tg = atoms_tables_global(a.pot, CT);  // always with gpair
   dim3 grid(nchunks, (ncand + CT - 1) / CT), block(BLOCK);
 #define MZ_LAUNCH(M, P, Q_, TG_, GP_)                                                              \
-  hipLaunchKernelGGL((k_atoms<BLOCK, PPL, MAXA, CT, M, P, Q_, TG_, GP_>), grid, block, dyn, st, a)
+  hipLaunchKernelGGL((k_atoms<BLOCK, PPL, MAXA, CT, M, P, Q_, TG_, GP_, 0>), grid, block, dyn, st, a)
   if (meam && a.pot.gpair) {
     if (point) { if (q) MZ_LAUNCH(true, true, true, true, true); else MZ_LAUNCH(true, true, false, true, true); }
     else { if (q) MZ_LAUNCH(true, false, true, true, true); else MZ_LAUNCH(true, false, false, true, true); }
@@ -832,13 +898,34 @@ static void launch_atoms_t(hipStream_t st, const AtomArgs& a, int nchunks, int n
 #undef MZ_LAUNCH
 }
 
+// jmeam styles: one candidate per workgroup, spline tables in global memory
+// (heads and the angular records staged), JM = 1 or 2 extra terms.
+template <int BLOCK, int PPL, int MAXA>
+static void launch_atoms_jm(hipStream_t st, const AtomArgs& a, int nchunks, int ncand) {
+  const int style = a.pot.style;
+  const bool point = style == kJMeam || style == kJMeam2;
+  const bool q = a.pot.quintic != 0;
+  const size_t dyn = atoms_dyn_lds(a.pot, 1);
+  dim3 grid(nchunks, ncand), block(BLOCK);
+#define MZ_LAUNCH(P, Q_, JM_) \
+  hipLaunchKernelGGL((k_atoms<BLOCK, PPL, MAXA, 1, true, P, Q_, true, false, JM_>), grid, block, dyn, st, a)
+  if (a.pot.njm == 1) {
+    if (point) { if (q) MZ_LAUNCH(true, true, 1); else MZ_LAUNCH(true, false, 1); }
+    else { if (q) MZ_LAUNCH(false, true, 1); else MZ_LAUNCH(false, false, 1); }
+  } else {
+    if (point) { if (q) MZ_LAUNCH(true, true, 2); else MZ_LAUNCH(true, false, 2); }
+    else { if (q) MZ_LAUNCH(false, true, 2); else MZ_LAUNCH(false, false, 2); }
+  }
+#undef MZ_LAUNCH
+}
+
 // MEAM potentials whose staged tables would take more than 16 KB of LDS per
 // workgroup read them from global memory instead (measured: the ternary
 // potential's 17.5 KB tables cost a candidate per workgroup and a workgroup
 // per CU; the one-element potential's 3.9 KB are faster staged).
 bool atoms_tables_global(const PotDesc& pot, int ct) {
   const bool meam = pot.ngco > 0;
-  return meam && (pot.gpair || (size_t)ct * (size_t)pot.table_bytes > 16 * 1024);
+  return meam && (pot.gpair || pot.njm > 0 || (size_t)ct * (size_t)pot.table_bytes > 16 * 1024);
 }
 
 size_t atoms_dyn_lds(const PotDesc& pot, int ct) {
@@ -848,16 +935,20 @@ size_t atoms_dyn_lds(const PotDesc& pot, int ct) {
   return (size_t)ct * ((size_t)pot.table_bytes + g);
 }
 
-size_t atoms_static_lds(int block, int ppl, int ct, bool gpair) {
+size_t atoms_static_lds(int block, int ppl, int ct, const PotDesc& pot) {
   const size_t P = (size_t)block * ppl;
   return P * 32 + (size_t)(ct > 1 ? ct - 1 : 1) * P * 8 + (size_t)ct * P * 8 + (size_t)ct * 64 * 8 * 2 + 65 * 4 +
-         (size_t)ct * 8 * (block / 64) * 8 + (gpair ? P * 4 : 4);
+         (size_t)ct * 8 * (block / 64) * 8 + (pot.gpair ? P * 4 : 4) +
+         (pot.njm > 0 ? (size_t)pot.njm * ct * P * 8 : 8);
 }
 
 int launch_atoms(hipStream_t st, const AtomArgs& a, int nchunks, int ncand, int block, int ppl, int ct) {
 #define MZ_SHAPE(B, P_)                                                       \
   if (block == B && ppl == P_) {                                             \
-    if (ct == 1) launch_atoms_t<B, P_, 64, 1>(st, a, nchunks, ncand);        \
+    if (a.pot.njm > 0) {                                                     \
+      if (ct != 1) return -1;                                                \
+      launch_atoms_jm<B, P_, 64>(st, a, nchunks, ncand);                     \
+    } else if (ct == 1) launch_atoms_t<B, P_, 64, 1>(st, a, nchunks, ncand); \
     else if (ct == 2) launch_atoms_t<B, P_, 64, 2>(st, a, nchunks, ncand);   \
     else if (ct == 4) launch_atoms_t<B, P_, 64, 4>(st, a, nchunks, ncand);   \
     else return -1;                                                          \

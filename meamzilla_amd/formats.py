@@ -19,7 +19,8 @@ from typing import List, Optional, Sequence
 
 import numpy as np
 
-STYLES = ("eam/spline", "meam/spline", "eam", "meam", "gmeam/spline", "gmeam", "pair/spline", "pair")
+STYLES = ("eam/spline", "meam/spline", "eam", "meam", "gmeam/spline", "gmeam", "pair/spline", "pair",
+          "jmeam/spline", "jmeam", "jmeam2/spline", "jmeam2")
 
 
 def n_pair_ij(nt: int) -> int:
@@ -28,7 +29,13 @@ def n_pair_ij(nt: int) -> int:
 
 def is_meam_style(style: str) -> bool:
     """Styles with the f and g (three-body) functions."""
-    return style.startswith(("meam", "gmeam"))
+    return style.startswith(("meam", "gmeam", "jmeam"))
+
+
+def jmeam_terms(style: str) -> int:
+    """Extra p p q energy terms: 1 for jmeam, 2 for jmeam2
+    (src/pots/pot_jmeam/pot_jmeam.cpp:30-47, src/pots/pot_jmeam2/pot_jmeam2.cpp:30-46)."""
+    return 2 if style.startswith("jmeam2") else (1 if style.startswith("jmeam") else 0)
 
 
 def is_pair_style(style: str) -> bool:
@@ -52,6 +59,7 @@ def function_counts(style: str, nt: int) -> List[int]:
     counts = [n_pair_ij(nt), nt, nt]
     if is_meam_style(style):
         counts += [n_pair_ij(nt), nt * n_pair_ij(nt) if is_gmeam_style(style) else nt]
+    counts += [n_pair_ij(nt), nt] * jmeam_terms(style)  # p Pair_ij, q Triplet_i (, p2, q2)
     return counts
 
 
@@ -124,6 +132,7 @@ class PotentialSpec:
     def max_rcut(self) -> float:
         counts = function_counts(self.style, self.ntypes)
         radial = [0] if is_pair_style(self.style) else [0, 1] + ([3] if is_meam_style(self.style) else [])
+        radial += [5, 7][:jmeam_terms(self.style)]
         r = 0.0
         off = 0
         for fi, c in enumerate(counts):

@@ -30,7 +30,7 @@ from typing import Callable, List, Optional, Tuple
 
 import numpy as np
 
-from .formats import Database, PotentialSpec, Spline, function_counts, is_meam_style
+from .formats import Database, PotentialSpec, Spline, function_counts, is_meam_style, jmeam_terms
 
 GPA_TO_EV_A3 = 1.0 / 160.21766208
 NOISE_E = 1e-3          # eV/atom
@@ -187,6 +187,13 @@ def make_potential(rng, style: str, nt: int, rc: float, nk: dict, u_range, basis
             s.y = s.y / m          # largest |f| knot is exactly 1
         spl += fs
         spl += [_flat(rng, -1.0, 1.0, nk["g"], -g_amp, g_amp, basis) for _ in range(counts[4])]
+    for m in range(jmeam_terms(style)):  # p (largest |knot| exactly 1) and q
+        ps = [_radial(rng, 1.8, rc, nk.get("p", nk["f"]), -1.0, 1.0, basis) for _ in range(counts[5 + 2 * m])]
+        mp = max(np.abs(s.y).max() for s in ps)
+        for s in ps:
+            s.y = s.y / mp
+        spl += ps
+        spl += [_flat(rng, -1.0, 1.0, nk.get("q", nk["g"]), -g_amp, g_amp, basis) for _ in range(counts[6 + 2 * m])]
     return PotentialSpec(style, nt, spl)
 
 

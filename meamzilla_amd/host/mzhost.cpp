@@ -39,6 +39,10 @@ int style_code(const std::string& s) {
   if (s == "gmeam") return MZ_STYLE_GMEAM;
   if (s == "pair/spline") return MZ_STYLE_PAIR_SPLINE;
   if (s == "pair") return MZ_STYLE_PAIR;
+  if (s == "jmeam/spline") return MZ_STYLE_JMEAM_SPLINE;
+  if (s == "jmeam") return MZ_STYLE_JMEAM;
+  if (s == "jmeam2/spline") return MZ_STYLE_JMEAM2_SPLINE;
+  if (s == "jmeam2") return MZ_STYLE_JMEAM2;
   throw Error("invalid potential style \"" + s + "\"");
 }
 
@@ -63,6 +67,9 @@ int splines_for_style(const std::string& style, int nt) {
   style_code(style);
   if (style.compare(0, 4, "pair") == 0) return npij;  // phi Pair_ij only
   if (style.compare(0, 5, "gmeam") == 0) return eam + npij + nt * npij;  // g Triplet_i_jk
+  // jmeam: p Pair_ij, q Triplet_i after g; jmeam2: also p2, q2
+  if (style.compare(0, 6, "jmeam2") == 0) return eam + npij + nt + 2 * (npij + nt);
+  if (style.compare(0, 5, "jmeam") == 0) return eam + npij + nt + (npij + nt);
   return (style.compare(0, 4, "meam") == 0) ? eam + npij + nt : eam;
 }
 
@@ -1071,7 +1078,9 @@ void write_data_files(const std::string& base, Potential& pot, const ConfigDb& d
   int natoms = 0;
   for (int c = 0; c < nc; ++c) natoms += db.natoms[c];
   const bool gmeam = pot.style().compare(0, 5, "gmeam") == 0;
-  const bool meam = gmeam || pot.style().compare(0, 4, "meam") == 0;
+  const bool jmeam = pot.style().compare(0, 5, "jmeam") == 0;
+  const int njm = jmeam ? (pot.style().compare(0, 6, "jmeam2") == 0 ? 2 : 1) : 0;
+  const bool meam = gmeam || jmeam || pot.style().compare(0, 4, "meam") == 0;
   using std::setw;
   using std::setprecision;
   if (!pair) {  // densities (src/pots/pot_eam/pot_eam.cpp:506-543)
@@ -1191,6 +1200,23 @@ void write_data_files(const std::string& base, Potential& pot, const ConfigDb& d
       o << "\nDummy constraints (|f_max(r)| = 1)\n";
       o << setw(15) << "punish error" << setw(15) << "|f_max(r)|" << "\n";
       o << std::fixed << setw(15) << setprecision(6) << ev[idx] * ev[idx] << setw(15) << std::abs(fmax) << "\n";
+      // jmeam: the same for the p (and p2) splines (src/pots/pot_jmeam/pot_jmeam.cpp:357-368,
+      // src/pots/pot_jmeam2/pot_jmeam2.cpp:381-405)
+      for (int m = 0; m < njm; ++m) {
+        ++idx;
+        const int p0 = f0 + nf + nt + m * (nf + nt);
+        double pmax = 0.0;
+        for (int q = p0; q < p0 + nf; ++q) {
+          double v = 0.0;
+          for (double y : pot.splines()[q].y)
+            if (std::abs(v) < std::abs(y)) v = y;
+          if (std::abs(pmax) < std::abs(v)) pmax = v;
+        }
+        const std::string nm = m == 0 ? "p" : "p2";
+        o << "\nDummy constraints (|" << nm << "_max(r)| = 1)\n";
+        o << setw(15) << "punish error" << setw(15) << ("|" + nm + "_max(r)|") << "\n";
+        o << std::fixed << setw(15) << setprecision(6) << ev[idx] * ev[idx] << setw(15) << std::abs(pmax) << "\n";
+      }
     } else {
       o << "\nDummy constraints (F'(n_avg) = 0)\n";
       o << setw(7) << "element" << setw(15) << "punish error" << setw(15) << "F'(n_avg)" << "\n";

@@ -30,7 +30,10 @@
  *     PotMEAM::compute src/pots/pot_meam/pot_meam.cpp:95-275, PotEAM::compute
  *     src/pots/pot_eam/pot_eam.cpp:129-251, PotPairSpline::fast_compute
  *     src/pots/pot_pair_spline/pot_pair_spline.cpp:74-141, PotPair::compute
- *     src/pots/pot_pair/pot_pair.cpp:73-144, accumulate_error src/potential.cpp:415-491;
+ *     src/pots/pot_pair/pot_pair.cpp:73-144, PotJMEAM::compute
+ *     src/pots/pot_jmeam/pot_jmeam.cpp:101-300, PotJMEAMSpline::fast_compute
+ *     src/pots/pot_jmeam_spline/pot_jmeam_spline.cpp:110-300, PotJMEAM2::compute
+ *     src/pots/pot_jmeam2/pot_jmeam2.cpp:107-330, accumulate_error src/potential.cpp:415-491;
  *   - alloy indices src/pot_fns.cpp:132-173.
  */
 #include "meamz_oracle.h"
@@ -59,13 +62,15 @@ typedef struct {
   int phi_idx, rho_idx, f_idx;
   int phi_knot, rho_knot, f_knot; /* -1 outside; spline styles hold the knot */
   double phi_shift, rho_shift, f_shift;
+  int p_idx[2], p_knot[2];        /* jmeam styles: p (and p2) */
+  double p_shift[2];
 } pair_t;
 
 struct orc_ctx {
   int style, nt, nspl, embed_extrap;
   int* basis; int* nknots; double* x0; double* xn; int* bc;
   int* koff;          /* knot-vector offset of each spline */
-  int fn_off[5];      /* first spline of each function */
+  int fn_off[9];      /* first spline of each function (phi rho U f g p q p2 q2) */
   int ncells, natoms;
   int* natoms_c; int* cell_of; int* types;
   double* force0; double* energy0; double* stress0; double* weight; double* eweight; double* sweight;
@@ -327,11 +332,19 @@ double orc_splint_point(int basis, int n, double x0, double xn, const int* bc, c
 /* ---------------------------------------------------------------- context */
 
 static int is_gmeam(const orc_ctx* c) { return c->style == ORC_GMEAM_SPLINE || c->style == ORC_GMEAM; }
-static int is_meam(const orc_ctx* c) { return c->style == ORC_MEAM_SPLINE || c->style == ORC_MEAM || is_gmeam(c); }
+/* extra p p q energy terms: 1 for jmeam, 2 for jmeam2 (src/pots/pot_jmeam/pot_jmeam.cpp:30-47,
+ * src/pots/pot_jmeam2/pot_jmeam2.cpp:30-46) */
+static int njm(const orc_ctx* c) {
+  return (c->style == ORC_JMEAM_SPLINE || c->style == ORC_JMEAM) ? 1
+         : (c->style == ORC_JMEAM2_SPLINE || c->style == ORC_JMEAM2) ? 2 : 0;
+}
+static int is_meam(const orc_ctx* c) {
+  return c->style == ORC_MEAM_SPLINE || c->style == ORC_MEAM || is_gmeam(c) || njm(c) > 0;
+}
 static int is_pair(const orc_ctx* c) { return c->style == ORC_PAIR_SPLINE || c->style == ORC_PAIR; }
 static int is_spline_style(const orc_ctx* c) {
   return c->style == ORC_EAM_SPLINE || c->style == ORC_MEAM_SPLINE || c->style == ORC_GMEAM_SPLINE ||
-         c->style == ORC_PAIR_SPLINE;
+         c->style == ORC_PAIR_SPLINE || c->style == ORC_JMEAM_SPLINE || c->style == ORC_JMEAM2_SPLINE;
 }
 
 /* PotFns::get_2body_alloy_idx Pair_ij, src/pot_fns.cpp:141-143 */
@@ -346,6 +359,46 @@ static const spl_t* g_spline(const orc_ctx* c, int ti, int tj, int tk) {
   const int nt = c->nt;
   const int idx = is_gmeam(c) ? ti * (nt * (nt + 1) / 2) + pair_ij(tj, tk, nt) : ti;
   return &c->spl[c->fn_off[4] + idx];
+}
+
+static int check_fn(const orc_ctx* c, const spl_t* s, double r, int* knot, double* shift);
+
+/* Terms a triplet carries: bit 0 the density term f_ij f_ik g, bit m (1, 2)
+ * the jmeam energy term p_ij p_ik q (p2 p2 q2).  Spline styles: the cosine
+ * lies inside the angular spline (get_knotshift); point styles: both legs lie
+ * inside the radial partner.  check_triplet src/pots/pot_meam/triplet_meam.cpp:48-71,
+ * src/pots/pot_meam_spline/triplet_meam_spline.cpp:49-73,
+ * src/pots/pot_jmeam/triplet_jmeam.cpp:48-88,
+ * src/pots/pot_jmeam_spline/triplet_jmeam_spline.cpp:49-88.  A triplet with
+ * no term is dropped.  (The jmeam spline styles evaluate both angular
+ * splines of a kept triplet at their stored knots, -1 included; here a term
+ * whose knot is -1 contributes nothing.) */
+static int triplet_terms(const orc_ctx* c, int ti, const pair_t* P, const pair_t* Q, double cs) {
+  const int nt = c->nt;
+  int terms = 0;
+  for (int m = 0; m <= njm(c); ++m) {
+    if (is_spline_style(c)) {
+      const spl_t* a = m == 0 ? g_spline(c, ti, c->types[P->j], c->types[Q->j]) : &c->spl[c->fn_off[4 + 2 * m] + ti];
+      double sh;
+      if (spl_knotshift(a, cs, &sh) != -1) terms |= 1 << m;
+    } else {
+      const int f = m == 0 ? 3 : 3 + 2 * m;
+      const spl_t* sj = &c->spl[c->fn_off[f] + pair_ij(ti, c->types[P->j], nt)];
+      const spl_t* sk = &c->spl[c->fn_off[f] + pair_ij(ti, c->types[Q->j], nt)];
+      if (!(P->r > sj->x[sj->n - 1] || Q->r > sk->x[sk->n - 1])) terms |= 1 << m;
+    }
+  }
+  return terms;
+}
+
+/* value and d/dcos of angular spline a at cs (knot form for spline styles) */
+static double ang_eval(const orc_ctx* c, const spl_t* a, double cs, double* d) {
+  if (is_spline_style(c)) {
+    double sh;
+    int kn = spl_knotshift(a, cs, &sh);
+    return spl_comb_knot(a, kn, sh, d);
+  }
+  return spl_comb_point(a, cs, d, NULL);
 }
 
 static void refresh_all(orc_ctx* c, const double* knots) {
@@ -373,8 +426,8 @@ orc_ctx* orc_create(const orc_layout* lay, const orc_db* db, char* err) {
   c->nspl = lay->nspl;
   c->embed_extrap = lay->embed_extrap;
   int nt = c->nt, npij = nt * (nt + 1) / 2;
-  int counts[5] = {npij, nt, nt, npij, is_gmeam(c) ? nt * npij : nt};
-  int nfn = is_meam(c) ? 5 : (is_pair(c) ? 1 : 3);
+  int counts[9] = {npij, nt, nt, npij, is_gmeam(c) ? nt * npij : nt, npij, nt, npij, nt};
+  int nfn = is_meam(c) ? 5 + 2 * njm(c) : (is_pair(c) ? 1 : 3);
   int tot = 0;
   for (int f = 0; f < nfn; ++f) { c->fn_off[f] = tot; tot += counts[f]; }
   if (tot != c->nspl) {
@@ -408,10 +461,10 @@ orc_ctx* orc_create(const orc_layout* lay, const orc_db* db, char* err) {
   }
   c->koff[c->nspl] = off;
 
-  /* max radial cutoff: phi, rho (, f) -- src/potential.cpp:73-82 */
+  /* max radial cutoff: phi, rho (, f (, p (, p2))) -- src/potential.cpp:73-82 */
   double max_rcut = 0.0;
-  int radial[3] = {0, 1, 3};
-  for (int q = 0; q < (is_meam(c) ? 3 : (is_pair(c) ? 1 : 2)); ++q) {
+  int radial[5] = {0, 1, 3, 5, 7};
+  for (int q = 0; q < (is_meam(c) ? 3 + njm(c) : (is_pair(c) ? 1 : 2)); ++q) {
     int f = radial[q];
     for (int s = c->fn_off[f]; s < c->fn_off[f] + counts[f]; ++s)
       if (c->xn[s] > max_rcut) max_rcut = c->xn[s];
@@ -510,6 +563,16 @@ orc_ctx* orc_create(const orc_layout* lay, const orc_db* db, char* err) {
                 int bad = check_fn(c, &c->spl[P.phi_idx], P.r, &P.phi_knot, &P.phi_shift);
                 if (!is_pair(c)) bad |= check_fn(c, &c->spl[P.rho_idx], P.r, &P.rho_knot, &P.rho_shift);
                 if (is_meam(c)) bad |= check_fn(c, &c->spl[P.f_idx], P.r, &P.f_knot, &P.f_shift);
+                int any_p = 0;
+                for (int m = 0; m < 2; ++m) {
+                  P.p_idx[m] = 0;
+                  P.p_knot[m] = -1;
+                  P.p_shift[m] = 0.0;
+                  if (m >= njm(c)) continue;
+                  P.p_idx[m] = c->fn_off[5 + 2 * m] + pair_ij(ti, tj, nt);
+                  bad |= check_fn(c, &c->spl[P.p_idx[m]], P.r, &P.p_knot[m], &P.p_shift[m]);
+                  any_p |= P.p_knot[m] != -1;
+                }
                 if (bad) {
                   if (err)
                     snprintf(err, 256, "Pair distance too short for potential (r=%g) cell=%d atom=%d neigh=%d", P.r,
@@ -517,7 +580,7 @@ orc_ctx* orc_create(const orc_layout* lay, const orc_db* db, char* err) {
                   orc_destroy(c);
                   return NULL;
                 }
-                if (P.phi_knot == -1 && P.rho_knot == -1 && (!is_meam(c) || P.f_knot == -1)) continue;
+                if (P.phi_knot == -1 && P.rho_knot == -1 && (!is_meam(c) || P.f_knot == -1) && !any_p) continue;
                 if (pass == 1) c->pairs[np] = P;
                 ++np;
               }
@@ -544,16 +607,8 @@ orc_ctx* orc_create(const orc_layout* lay, const orc_db* db, char* err) {
         for (long long q = p + 1; q < e; ++q) {
           const pair_t* P = &c->pairs[p];
           const pair_t* Q = &c->pairs[q];
-          const spl_t* g = g_spline(c, ti, c->types[P->j], c->types[Q->j]);
-          if (is_spline_style(c)) {
-            double cs = P->dist[0] * Q->dist[0] + P->dist[1] * Q->dist[1] + P->dist[2] * Q->dist[2];
-            double sh;
-            if (spl_knotshift(g, cs, &sh) != -1) ++nt_a;
-          } else {
-            const spl_t* fj = &c->spl[P->f_idx];
-            const spl_t* fk = &c->spl[Q->f_idx];
-            if (!(P->r > fj->x[fj->n - 1] || Q->r > fk->x[fk->n - 1])) ++nt_a;
-          }
+          double cs = P->dist[0] * Q->dist[0] + P->dist[1] * Q->dist[1] + P->dist[2] * Q->dist[2];
+          if (triplet_terms(c, ti, P, Q, cs)) ++nt_a;
         }
     }
     c->ntrip_atom[a] = nt_a;
@@ -578,7 +633,8 @@ void orc_counts(const orc_ctx* c, long long* npairs, long long* ntriplets) {
 
 int orc_errvec_size(const orc_ctx* c) {
   if (is_pair(c)) return 7 * c->ncells + 3 * c->natoms;
-  return 7 * c->ncells + 3 * c->natoms + c->ncells + (is_gmeam(c) ? c->nt * (c->nt + 1) / 2 : is_meam(c) ? 1 : c->nt);
+  return 7 * c->ncells + 3 * c->natoms + c->ncells +
+         (is_gmeam(c) ? c->nt * (c->nt + 1) / 2 : is_meam(c) ? 1 + njm(c) : c->nt);
 }
 
 /* ------------------------------------------------------------ objective core */
@@ -589,6 +645,7 @@ typedef struct {
   double* force;      /* [natoms*3] */
   double* constraint; /* [ncells] */
   double* drho; double* f; double* df; /* per-pair scratch */
+  double* p[2]; double* dp[2];         /* jmeam styles: p (and p2) per pair */
 } accum_t;
 
 static double eval_fn(const orc_ctx* c, const spl_t* s, int knot, double shift, double r, double* grad) {
@@ -664,6 +721,14 @@ static void atom_body(const orc_ctx* c, int a, accum_t* acc) {
         acc->df[p] = 0.0;
       }
     }
+    for (int m = 0; m < njm(c); ++m) {
+      if (P->p_knot[m] != -1) {
+        acc->p[m][p] = eval_fn(c, &c->spl[P->p_idx[m]], P->p_knot[m], P->p_shift[m], P->r, &acc->dp[m][p]);
+      } else {
+        acc->p[m][p] = 0.0;
+        acc->dp[m][p] = 0.0;
+      }
+    }
   }
   const int ti = c->types[a];
   if (meam) {
@@ -672,20 +737,12 @@ static void atom_body(const orc_ctx* c, int a, accum_t* acc) {
         const pair_t* P = &c->pairs[p];
         const pair_t* Q = &c->pairs[q];
         double cs = P->dist[0] * Q->dist[0] + P->dist[1] * Q->dist[1] + P->dist[2] * Q->dist[2];
-        const spl_t* g = g_spline(c, ti, c->types[P->j], c->types[Q->j]);
-        double gv, dg;
-        if (is_spline_style(c)) {
-          double sh;
-          int kn = spl_knotshift(g, cs, &sh);
-          if (kn == -1) continue;
-          gv = spl_comb_knot(g, kn, sh, &dg);
-        } else {
-          const spl_t* fj = &c->spl[P->f_idx];
-          const spl_t* fk = &c->spl[Q->f_idx];
-          if (P->r > fj->x[fj->n - 1] || Q->r > fk->x[fk->n - 1]) continue;
-          gv = spl_comb_point(g, cs, &dg, NULL);
-        }
-        rho_val += acc->f[p] * acc->f[q] * gv;
+        const int terms = triplet_terms(c, ti, P, Q, cs);
+        double dg;
+        if (terms & 1) rho_val += acc->f[p] * acc->f[q] * ang_eval(c, g_spline(c, ti, c->types[P->j], c->types[Q->j]), cs, &dg);
+        for (int m = 0; m < njm(c); ++m)
+          if (terms >> (m + 1) & 1)
+            acc->energy[cc] += acc->p[m][p] * acc->p[m][q] * ang_eval(c, &c->spl[c->fn_off[6 + 2 * m] + ti], cs, &dg);
       }
   }
   const spl_t* U = &c->spl[c->fn_off[2] + c->types[a]];
@@ -715,22 +772,21 @@ static void atom_body(const orc_ctx* c, int a, accum_t* acc) {
         const pair_t* P = &c->pairs[p];
         const pair_t* Q = &c->pairs[q];
         double cs = P->dist[0] * Q->dist[0] + P->dist[1] * Q->dist[1] + P->dist[2] * Q->dist[2];
-        const spl_t* g = g_spline(c, ti, c->types[P->j], c->types[Q->j]);
-        double gv, dg;
-        if (is_spline_style(c)) {
-          double sh;
-          int kn = spl_knotshift(g, cs, &sh);
-          if (kn == -1) continue;
-          gv = spl_comb_knot(g, kn, sh, &dg);
-        } else {
-          const spl_t* fj = &c->spl[P->f_idx];
-          const spl_t* fk = &c->spl[Q->f_idx];
-          if (P->r > fj->x[fj->n - 1] || Q->r > fk->x[fk->n - 1]) continue;
-          gv = spl_comb_point(g, cs, &dg, NULL);
-        }
+        const int terms = triplet_terms(c, ti, P, Q, cs);
+        if (!terms) continue;
+        double gv = 0.0, dg = 0.0;
+        if (terms & 1) gv = ang_eval(c, g_spline(c, ti, c->types[P->j], c->types[Q->j]), cs, &dg);
         double dV3j = gv * acc->df[p] * acc->f[q] * dF;
         double dV3k = gv * acc->f[p] * acc->df[q] * dF;
         double V3 = dg * acc->f[p] * acc->f[q] * dF;
+        /* jmeam: + q p'_j p_k, + q p_j p'_k, + q' p_j p_k (src/pots/pot_jmeam/pot_jmeam.cpp:263-265) */
+        for (int m = 0; m < njm(c); ++m) {
+          double qv = 0.0, dq = 0.0;
+          if (terms >> (m + 1) & 1) qv = ang_eval(c, &c->spl[c->fn_off[6 + 2 * m] + ti], cs, &dq);
+          dV3j += qv * acc->dp[m][p] * acc->p[m][q];
+          dV3k += qv * acc->p[m][p] * acc->dp[m][q];
+          V3 += dq * acc->p[m][p] * acc->p[m][q];
+        }
         double vlj = V3 * P->invr;
         double vlk = V3 * Q->invr;
         double vv3j = dV3j - vlj * cs;
@@ -808,17 +864,21 @@ static double finish(const orc_ctx* c, const accum_t* acc, double* errvec, doubl
       ++cnt;
     }
   } else if (is_meam(c)) {
-    /* PotFns::get_max_y_mag src/pot_fns.cpp:100-111 over the f splines */
+    /* PotFns::get_max_y_mag src/pot_fns.cpp:100-111 over the f splines, then
+     * (jmeam) over the p and p2 splines, src/pots/pot_jmeam2/pot_jmeam2.cpp:308-324 */
     int npij = c->nt * (c->nt + 1) / 2;
-    double mx = 0.0;
-    for (int s = c->fn_off[3]; s < c->fn_off[3] + npij; ++s) {
-      double v = spl_max_y_mag(&c->spl[s]);
-      if (fabs(mx) < fabs(v)) mx = v;
+    for (int grp = 0; grp <= njm(c); ++grp) {
+      const int f0 = c->fn_off[grp == 0 ? 3 : 3 + 2 * grp];
+      double mx = 0.0;
+      for (int s = f0; s < f0 + npij; ++s) {
+        double v = spl_max_y_mag(&c->spl[s]);
+        if (fabs(mx) < fabs(v)) mx = v;
+      }
+      double m = fabs(mx);
+      double pe = DUMMY_WEIGHT * 25. * (1.0 - m) * (1.0 - m);
+      F += pe * pe;
+      if (errvec) errvec[cnt + grp] = pe;
     }
-    double m = fabs(mx);
-    double pe = DUMMY_WEIGHT * 25. * (1.0 - m) * (1.0 - m);
-    F += pe * pe;
-    if (errvec) errvec[cnt] = pe;
   } else {
     for (int t = 0; t < c->nt; ++t) {
       const spl_t* U = &c->spl[c->fn_off[2] + t];
@@ -841,11 +901,16 @@ static void accum_alloc(const orc_ctx* c, accum_t* acc) {
   acc->drho = (double*)calloc(np, sizeof(double));
   acc->f = (double*)calloc(np, sizeof(double));
   acc->df = (double*)calloc(np, sizeof(double));
+  for (int m = 0; m < 2; ++m) {
+    acc->p[m] = (double*)calloc(np, sizeof(double));
+    acc->dp[m] = (double*)calloc(np, sizeof(double));
+  }
 }
 
 static void accum_free(accum_t* acc) {
   free(acc->energy); free(acc->stress); free(acc->force); free(acc->constraint);
   free(acc->drho); free(acc->f); free(acc->df);
+  for (int m = 0; m < 2; ++m) { free(acc->p[m]); free(acc->dp[m]); }
 }
 
 double orc_eval(orc_ctx* c, const double* knots, double* errvec, double* cell_energy, double* cell_stress,
@@ -951,19 +1016,8 @@ void orc_densities(orc_ctx* c, const double* knots, double* dens) {
           const pair_t* Q = &c->pairs[q];
           const spl_t* g = g_spline(c, c->types[a], c->types[P->j], c->types[Q->j]);
           double cs = P->dist[0] * Q->dist[0] + P->dist[1] * Q->dist[1] + P->dist[2] * Q->dist[2];
-          double gv;
-          if (is_spline_style(c)) {
-            double sh;
-            int kn = spl_knotshift(g, cs, &sh);
-            if (kn == -1) continue;
-            gv = spl_comb_knot(g, kn, sh, &tmp);
-          } else {
-            const spl_t* fj = &c->spl[P->f_idx];
-            const spl_t* fk = &c->spl[Q->f_idx];
-            if (P->r > fj->x[fj->n - 1] || Q->r > fk->x[fk->n - 1]) continue;
-            gv = spl_comb_point(g, cs, &tmp, NULL);
-          }
-          d += f[p] * f[q] * gv;
+          if (!(triplet_terms(c, c->types[a], P, Q, cs) & 1)) continue;
+          d += f[p] * f[q] * ang_eval(c, g, cs, &tmp);
         }
     }
     dens[a] = d;

@@ -22,13 +22,18 @@ enum {
   ORC_GMEAM_SPLINE = 4,
   ORC_GMEAM = 5,
   ORC_PAIR_SPLINE = 6,
-  ORC_PAIR = 7
+  ORC_PAIR = 7,
+  ORC_JMEAM_SPLINE = 8,
+  ORC_JMEAM = 9,
+  ORC_JMEAM2_SPLINE = 10,
+  ORC_JMEAM2 = 11
 };
 
 /* One potential's spline layout.  Splines are listed in file order
  * (phi Pair_ij, rho Pair_j, U Pair_i [, f Pair_ij, g Triplet_i; gmeam styles:
- * g Triplet_i_jk, ntypes * ntypes (ntypes + 1) / 2 splines]; pair styles: phi
- * Pair_ij only).
+ * g Triplet_i_jk, ntypes * ntypes (ntypes + 1) / 2 splines]; jmeam styles add
+ * p Pair_ij, q Triplet_i, jmeam2 also p2 Pair_ij, q2 Triplet_i; pair styles:
+ * phi Pair_ij only).
  * bc holds two ints per spline: -1 natural, 0 free slope, 1 fixed slope. */
 typedef struct {
   int style;

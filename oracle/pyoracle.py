@@ -15,7 +15,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libmeamz_oracle.so")
 STYLE_CODE = {"eam/spline": 0, "meam/spline": 1, "eam": 2, "meam": 3, "gmeam/spline": 4, "gmeam": 5,
-              "pair/spline": 6, "pair": 7}
+              "pair/spline": 6, "pair": 7, "jmeam/spline": 8, "jmeam": 9, "jmeam2/spline": 10, "jmeam2": 11}
 
 
 def build() -> str:

@@ -189,6 +189,7 @@ def main():
 
     gmeam_cases()
     pair_cases()
+    jmeam_cases()
 
 
 def gmeam_cases():
@@ -238,9 +239,40 @@ def pair_cases():
             note="one atom per cell: only self-image pairs (half weight)")
 
 
+def jmeam_cases():
+    """jmeam styles: MEAM plus p_ij p_ik q(cos) energy terms (one for jmeam,
+    two for jmeam2) outside the embedding, and one |p_max| = 1 penalty per p
+    function.  ``python tests/golden/make_golden.py jmeam`` writes only these."""
+    rng = np.random.Generator(np.random.PCG64(7272))
+    w = gen.workload(2, scale=8 / 512)
+    small = [c for c in range(w.db.ncells) if w.db.natoms[c] <= 200][:6]
+    nk = dict(phi=10, rho=10, U=10, f=10, g=10, p=9, q=8)
+    pot = gen.make_potential(rng, "jmeam/spline", 2, 5.0, nk, (-1.0, 3.0))
+    truth = gen.make_potential(rng, "jmeam/spline", 2, 5.0, nk, (-1.0, 3.0))
+    _finish("jmeam_spline_binary", pot, _slice(w, small[:4]), truth=truth,
+            note="binary jmeam/spline (p, q terms)")
+    pot = pot.copy()
+    pot.style = "jmeam"
+    pot.splines[8].xn = 4.2           # f_AB ends early: some triplets carry the q term only
+    pot.splines[14].xn = 3.9          # p_BB ends early: some carry the g term only
+    _finish("jmeam_point_binary", pot, _slice(w, small[:3]), truth=truth,
+            note="binary jmeam (point style), f_AB on [1.8,4.2], p_BB on [1.8,3.9]")
+    pot2 = gen.make_potential(rng, "jmeam2/spline", 2, 5.0, nk, (-1.0, 3.0), basis=3)
+    truth2 = gen.make_potential(rng, "jmeam2/spline", 2, 5.0, nk, (-1.0, 3.0))
+    _finish("jmeam2_spline_binary", pot2, _slice(w, small[1:4]), truth=truth2,
+            note="binary jmeam2/spline (p, q and p2, q2 terms)")
+    pot2 = pot2.copy()
+    pot2.style = "jmeam2"
+    pot2.splines[17].xn = 4.4         # p2_AA ends early
+    _finish("jmeam2_point_binary", pot2, _slice(w, small[:2]), truth=truth2,
+            note="binary jmeam2 (point style), p2_AA on [1.8,4.4]")
+
+
 if __name__ == "__main__":
     if len(sys.argv) > 1 and sys.argv[1] == "gmeam":
         gmeam_cases()
+    elif len(sys.argv) > 1 and sys.argv[1] == "jmeam":
+        jmeam_cases()
     elif len(sys.argv) > 1 and sys.argv[1] == "pair":
         pair_cases()
     else:

@@ -22,7 +22,8 @@ import tempfile
 ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
 GOLDEN = os.path.join(ROOT, "tests", "golden")
 REF = os.path.join(ROOT, "oracle", "_ref", "meamz_ref")
-CASES = ["eam_spline_c0", "meam_spline_q5", "meam_spline_binary", "gmeam_spline_binary", "pair_spline_binary"]
+CASES = ["eam_spline_c0", "meam_spline_q5", "meam_spline_binary", "gmeam_spline_binary", "pair_spline_binary",
+         "jmeam2_spline_binary"]
 SUFFIXES = [".energy", ".stress", ".force", ".error", ".punish", ".rho", ".lmp"]
 
 

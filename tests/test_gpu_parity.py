@@ -36,6 +36,38 @@ def test_gpu_matches_reference_golden(case):
         assert rel_err(obj.densities(), exp["densities"]) < TOL
 
 
+@pytest.mark.parametrize("style", ["jmeam/spline", "jmeam", "jmeam2/spline", "jmeam2"])
+def test_gpu_jmeam_styles_match_oracle(style):
+    """jmeam styles on the binary workload slice (a few thousand atoms): the
+    p_ij p_ik q terms ride the forward walk beside g; plain and batched calls
+    against the oracle."""
+    _say("jmeam style", style)
+    w = gen.workload(2, scale=24 / 512)
+    rng = np.random.Generator(np.random.PCG64(91))
+    nk = dict(phi=12, rho=12, U=10, f=12, g=10, p=11, q=9)
+    w.pot = gen.make_potential(rng, style, 2, 5.0, nk, (-1.0, 3.0))
+    w.truth = gen.make_potential(rng, style, 2, 5.0, nk, (-1.0, 3.0))
+    if not style.endswith("spline"):
+        w.pot.splines[13].xn = 4.3  # p_AB ends early: triplets with the g term only
+    gen.add_targets(w, pyoracle.evaluator)
+    o = pyoracle.Oracle(w.pot, w.db)
+    obj = Objective(w.pot, w.db)
+    assert (obj.info.npairs, obj.info.ntriplets) == o.counts()
+    e = o.eval()
+    r = obj.compute()
+    assert abs(r["F"] - e["F"]) <= TOL * abs(e["F"])
+    for k in ("errvec", "cell_energy", "cell_stress", "forces"):
+        assert rel_err(r[k], e[k]) < TOL, k
+    assert rel_err(obj.densities(), o.densities()) < TOL
+    kv = w.pot.knot_vector()
+    ks = np.stack([kv, kv * 1.01])
+    Fb, Eb = obj.residuals_batch(ks)
+    for n in range(2):
+        en = o.eval(ks[n])
+        assert abs(Fb[n] - en["F"]) <= TOL * abs(en["F"])
+        assert rel_err(Eb[n], en["errvec"]) < TOL
+
+
 @pytest.mark.parametrize("style", ["pair/spline", "pair"])
 def test_gpu_pair_styles_match_oracle(style):
     """Pair styles on the binary workload slice: the device walks the full pair

@@ -13,7 +13,8 @@ import pytest
 from meamzilla_amd import _lib, formats
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
-CASES = ["eam_spline_c0", "meam_spline_q5", "meam_spline_binary", "gmeam_spline_binary", "pair_spline_binary"]
+CASES = ["eam_spline_c0", "meam_spline_q5", "meam_spline_binary", "gmeam_spline_binary", "pair_spline_binary",
+         "jmeam2_spline_binary"]
 RNG = np.random.Generator(np.random.PCG64(31))
 
 
