@@ -90,17 +90,27 @@ typedef struct {
   long long ntriplets;    /* sum over atoms of kept j<k neighbour pairs */
   int nchunks;            /* work units of the atom kernel */
   int device;             /* HIP device ordinal */
+  double list_ms;         /* wall time of the pair-list build inside mz_create */
+  int list_device;        /* 1: pair lists built on the device (mz_create), 0: on the host */
 } mz_info_t;
 
 typedef struct mz_ctx mz_ctx;
 
-/* Builds the pair lists on the host (the reference's Config::init /
- * find_pairs / find_triplets, src/config.cpp:46-285, and AtomVec::init,
- * src/atom_vec.cpp:41-140, re-decomposed into atom chunks), then uploads the
- * geometry, lists and targets to the current HIP device.
+/* Builds the pair lists on the current HIP device (the reference's
+ * Config::init / find_pairs / find_triplets, src/config.cpp:46-285, and
+ * AtomVec::init, src/atom_vec.cpp:41-140, re-decomposed into atom chunks: a
+ * count pass and a fill pass, one thread per atom, the list left resident),
+ * counts triplets and uploads the geometry and targets.
  * Replaces: Config::init src/config.cpp:46-119, Config::find_pairs :143-234,
  * Config::find_triplets :240-285, Cell::read_cell/find_scale src/cell.cpp:39-170. */
 int mz_create(const mz_layout* layout, const mz_db* db, mz_ctx** out);
+
+/* mz_create with the pair lists built on the host by the same rules, one host
+ * thread per cell (the reference's own route, src/config.cpp:143-234).  The
+ * lists, and so every result, are identical to mz_create's; it serves as the
+ * cross-check of the device builder.
+ * No reference counterpart beyond mz_create's. */
+int mz_create_host_lists(const mz_layout* layout, const mz_db* db, mz_ctx** out);
 void mz_destroy(mz_ctx* ctx);
 int mz_info(const mz_ctx* ctx, mz_info_t* out);
 const char* mz_last_error(void);

@@ -41,7 +41,7 @@ class _Db(C.Structure):
 class Info(C.Structure):
     _fields_ = [("ncells", C.c_int), ("natoms", C.c_int), ("nspl", C.c_int), ("nknot", C.c_int),
                 ("nerr", C.c_int), ("npairs", C.c_longlong), ("ntriplets", C.c_longlong),
-                ("nchunks", C.c_int), ("device", C.c_int)]
+                ("nchunks", C.c_int), ("device", C.c_int), ("list_ms", C.c_double), ("list_device", C.c_int)]
 
 
 class Tile(C.Structure):
@@ -74,6 +74,7 @@ def lib() -> C.CDLL:
     vp = C.c_void_p
     L.mz_last_error.restype = C.c_char_p
     L.mz_create.argtypes = [vp, vp, C.POINTER(vp)]
+    L.mz_create_host_lists.argtypes = [vp, vp, C.POINTER(vp)]
     L.mz_destroy.argtypes = [vp]
     L.mz_info.argtypes = [vp, C.POINTER(Info)]
     L.mz_objective.argtypes = [vp, dp, dp, dp, dp, dp, dp]
@@ -117,9 +118,11 @@ class Objective:
     Mirrors the reference's Potential::compute(comm, ErrorVec*) contract:
     ``compute(knots, want_errvec)`` returns the objective and, on request, the
     residual vector; per-cell energies / stresses and forces are returned too.
+    ``host_lists`` builds the pair lists on the host (mz_create_host_lists)
+    instead of on the device; the lists are the same.
     """
 
-    def __init__(self, pot: PotentialSpec, db: Database):
+    def __init__(self, pot: PotentialSpec, db: Database, host_lists: bool = False):
         L = lib()
         self.pot = pot
         keep = {}
@@ -142,7 +145,8 @@ class Objective:
                                                           "stress0", "weight", "eweight", "sweight")],
                   float(db.energy_weight), float(db.stress_weight))
         ctx = C.c_void_p()
-        _check(L.mz_create(C.byref(lay), C.byref(cdb), C.byref(ctx)))
+        create = L.mz_create_host_lists if host_lists else L.mz_create
+        _check(create(C.byref(lay), C.byref(cdb), C.byref(ctx)))
         self._ctx = ctx
         self.info = Info()
         _check(L.mz_info(ctx, C.byref(self.info)))

@@ -82,6 +82,8 @@ struct mz_ctx {
   mz::PotDesc pot{};
   int style = 0, nt = 0, nspl = 0;
   bool meam = false, point = false, pair = false;
+  double list_ms = 0.0;  // wall time of the pair-list build in mz_create
+  int list_device = 0;   // 1: lists built on the device
   int nfpen = 0;  // f-max penalty terms: 1 (meam), one per f spline (gmeam)
   std::vector<int> basis, nknots, bc, koff, coef_off;
   std::vector<double> x0, xn;

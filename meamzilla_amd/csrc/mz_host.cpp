@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <atomic>
 #include <cmath>
 #include <cstdio>
@@ -16,6 +17,7 @@
 #include "mz_kernels.h"
 #include "mz_adjoint.h"
 #include "mz_ctx.h"
+#include "mz_list.h"
 
 using mz::DevBuf;
 using mz::Grid;
@@ -611,7 +613,7 @@ extern "C" {
 
 const char* mz_last_error(void) { return g_err.c_str(); }
 
-int mz_create(const mz_layout* layout, const mz_db* db, mz_ctx** out) {
+static int create_impl(const mz_layout* layout, const mz_db* db, bool device_lists, mz_ctx** out) {
   if (!out || !layout || !db) return fail(MZ_ERR_ARG, "null argument");
   *out = nullptr;
   if (db->ncells < 1) return fail(MZ_ERR_ARG, "database has no cells");
@@ -665,45 +667,63 @@ int mz_create(const mz_layout* layout, const mz_db* db, mz_ctx** out) {
     for (int s = c->pot.fn_off[radial[q]]; s < c->pot.fn_off[radial[q]] + counts[radial[q]]; ++s)
       max_rcut = std::max(max_rcut, c->xn[s]);
 
-  // pair lists, cells spread over host threads
-  std::vector<CellPairs> cp(c->ncells);
-  {
-    unsigned nth = std::max(1u, std::min(32u, std::thread::hardware_concurrency()));
-    std::atomic<int> next{0};
-    std::vector<std::thread> th;
-    for (unsigned t = 0; t < nth; ++t)
-      th.emplace_back([&]() {
-        for (int k; (k = next.fetch_add(1)) < c->ncells;)
-          cell_pairs(c, k, db->box + 9 * k, db->pos, max_rcut, c->vol[k], cp[k]);
-      });
-    for (auto& t : th) t.join();
-  }
-  c->atom_poff.assign(na + 1, 0);
-  long long np = 0;
-  for (int k = 0; k < c->ncells; ++k) {
-    if (!cp[k].err.empty()) {
-      std::string e = cp[k].err;
-      delete c;
-      return fail(MZ_ERR_GEOMETRY, e);
-    }
-    for (int a = 0; a < c->natoms_c[k]; ++a) {
-      c->atom_poff[c->abase[k] + a] = (int)np;
-      np += (long long)cp[k].per_atom[a].size();
-    }
-  }
-  if (np > 2000000000LL) {
+  // the HIP device and stream first: the device list builder runs on them
+  if (hipGetDevice(&c->device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess) {
     delete c;
-    return fail(MZ_ERR_CAPACITY, "more than 2e9 pairs");
+    return fail(MZ_ERR_HIP, "no HIP device available");
   }
-  c->atom_poff[na] = (int)np;
-  c->npairs = np;
-  c->pairs.resize(np);
-  for (int k = 0; k < c->ncells; ++k)
-    for (int a = 0; a < c->natoms_c[k]; ++a) {
-      auto& v = cp[k].per_atom[a];
-      std::copy(v.begin(), v.end(), c->pairs.begin() + c->atom_poff[c->abase[k] + a]);
-      std::vector<uint2>().swap(v);
+  const auto t_list = std::chrono::steady_clock::now();
+  if (device_lists) {
+    std::string e;
+    rc = mz::build_pairs_device(c, db->pos, db->box, max_rcut, e);
+    if (rc) {
+      mz_destroy(c);
+      return fail(rc, e);
     }
+  } else {
+    // pair lists, cells spread over host threads
+    std::vector<CellPairs> cp(c->ncells);
+    {
+      unsigned nth = std::max(1u, std::min(32u, std::thread::hardware_concurrency()));
+      std::atomic<int> next{0};
+      std::vector<std::thread> th;
+      for (unsigned t = 0; t < nth; ++t)
+        th.emplace_back([&]() {
+          for (int k; (k = next.fetch_add(1)) < c->ncells;)
+            cell_pairs(c, k, db->box + 9 * k, db->pos, max_rcut, c->vol[k], cp[k]);
+        });
+      for (auto& t : th) t.join();
+    }
+    c->atom_poff.assign(na + 1, 0);
+    long long np = 0;
+    for (int k = 0; k < c->ncells; ++k) {
+      if (!cp[k].err.empty()) {
+        std::string e = cp[k].err;
+        mz_destroy(c);
+        return fail(MZ_ERR_GEOMETRY, e);
+      }
+      for (int a = 0; a < c->natoms_c[k]; ++a) {
+        c->atom_poff[c->abase[k] + a] = (int)np;
+        np += (long long)cp[k].per_atom[a].size();
+      }
+    }
+    if (np > 2000000000LL) {
+      mz_destroy(c);
+      return fail(MZ_ERR_CAPACITY, "more than 2e9 pairs");
+    }
+    c->atom_poff[na] = (int)np;
+    c->npairs = np;
+    c->pairs.resize(np);
+    for (int k = 0; k < c->ncells; ++k)
+      for (int a = 0; a < c->natoms_c[k]; ++a) {
+        auto& v = cp[k].per_atom[a];
+        std::copy(v.begin(), v.end(), c->pairs.begin() + c->atom_poff[c->abase[k] + a]);
+        std::vector<uint2>().swap(v);
+      }
+  }
+  c->list_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_list).count();
+  c->list_device = device_lists ? 1 : 0;
+  const long long np = c->npairs;
   c->ntrip_atom.assign(na, 0);
   {
     unsigned nth = std::max(1u, std::min(32u, std::thread::hardware_concurrency()));
@@ -729,16 +749,13 @@ int mz_create(const mz_layout* layout, const mz_db* db, mz_ctx** out) {
     for (long long q = 0; q < np; ++q) c->in_list[fill[c->pairs[q].x]++] = (int)q;
   }
 
-  // device upload
-  if (hipGetDevice(&c->device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess) {
-    delete c;
-    return fail(MZ_ERR_HIP, "no HIP device available");
-  }
+  // device upload (the device builder already placed the pair list and offsets)
   c->energy_weight = db->energy_weight;
   c->stress_weight = db->stress_weight;
   std::vector<double> s0(db->stress0, db->stress0 + 6 * (size_t)c->ncells);
-  bool bad = c->d_atom_poff.upload(c->atom_poff.data(), c->atom_poff.size()) ||
-             c->d_pairs.upload(c->pairs.data(), c->pairs.size()) || c->d_in_off.upload(c->in_off.data(), c->in_off.size()) ||
+  bool bad = (!device_lists && (c->d_atom_poff.upload(c->atom_poff.data(), c->atom_poff.size()) ||
+                                 c->d_pairs.upload(c->pairs.data(), c->pairs.size()))) ||
+             c->d_in_off.upload(c->in_off.data(), c->in_off.size()) ||
              c->d_in_list.upload(c->in_list.data(), c->in_list.size()) ||
              c->d_pos.upload(db->pos, 3 * (size_t)na) || c->d_box.upload(db->box, 9 * (size_t)c->ncells) ||
              c->d_types.upload(c->types.data(), na) || c->d_cell_of.upload(c->cell_of.data(), na) ||
@@ -768,6 +785,12 @@ int mz_create(const mz_layout* layout, const mz_db* db, mz_ctx** out) {
   }
   *out = c;
   return MZ_OK;
+}
+
+int mz_create(const mz_layout* layout, const mz_db* db, mz_ctx** out) { return create_impl(layout, db, true, out); }
+
+int mz_create_host_lists(const mz_layout* layout, const mz_db* db, mz_ctx** out) {
+  return create_impl(layout, db, false, out);
 }
 
 void mz_destroy(mz_ctx* c) {
@@ -805,6 +828,8 @@ int mz_info(const mz_ctx* c, mz_info_t* out) {
   out->ntriplets = c->ntriplets;
   out->nchunks = (int)c->chunks.size();
   out->device = c->device;
+  out->list_ms = c->list_ms;
+  out->list_device = c->list_device;
   return MZ_OK;
 }
 

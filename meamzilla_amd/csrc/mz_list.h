@@ -1,0 +1,16 @@
+// Device-side pair-list build (see mz_list.hip).
+#pragma once
+
+#include <string>
+
+struct mz_ctx;
+
+namespace mz {
+
+// Builds c->atom_poff / c->pairs (host copies) and c->d_atom_poff / c->d_pairs
+// for the cells, types, spline grids and radial cutoff already set in c; also
+// uploads positions, boxes, types and the spline end points.  Returns MZ_OK or
+// an MZ_ERR_* code with the message in err.
+int build_pairs_device(mz_ctx* c, const double* pos, const double* box, double max_rcut, std::string& err);
+
+}  // namespace mz
