@@ -37,7 +37,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 from meamzilla_amd import formats, gen  # noqa: E402
-from meamzilla_amd._lib import Objective  # noqa: E402
+from meamzilla_amd._lib import Objective, probe_peaks  # noqa: E402
 
 # Algorithmic work per unit, counted from the kernels' arithmetic (cubic
 # splines, knot-style evaluation; one FMA = 2 FLOP, sqrt and divide = 1).
@@ -49,6 +49,8 @@ ADJ_FLOP_PER_PAIR = 162      # k_adjoint, per pair (geometry, rho/f, duals, phi/
 # pair; position 24 + type 4 + pair offset 4 + own-force sum 24 per atom.
 FWD_BYTES_PER_PAIR = 32
 FWD_BYTES_PER_ATOM = 56
+RESID_BYTES_PER_PAIR = 28    # k_resid: pair-force gather (24 B) + incoming index (4 B) per full-list pair
+RESID_BYTES_PER_ATOM = 76    # k_resid: own force sum, target force, force out, lambda out, index, type/cell
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
 
@@ -95,7 +97,13 @@ def cpu_baseline(w, seconds: float, ncoeff: int, ntriplets: int) -> dict:
         subset_note = (f"; run on the first {m} of the workload's cells ({sub} of {ntriplets} triplets) and scaled "
                        f"to the whole workload by triplet count")
         log(f"cpu baseline on {m} cells ({sub} triplets)")
-    ref = os.path.join(ROOT, "oracle", "_ref", "meamz_ref")
+    # the reference as shipped (-Ofast -march=corei7 -funroll-all-loops) where
+    # built, else its IEEE -O2 build
+    ref = os.path.join(ROOT, "oracle", "_ref", "meamz_ref_fast")
+    flags = "-Ofast -march=corei7 -funroll-all-loops, the shipped flags"
+    if not os.path.exists(ref):
+        ref = os.path.join(ROOT, "oracle", "_ref", "meamz_ref")
+        flags = "-O2 IEEE build"
     mpiexec = shutil.which("mpiexec") or "/opt/conda/bin/mpiexec"
     if os.path.exists(ref) and os.path.exists(mpiexec):
         tmp = tempfile.mkdtemp(prefix="meamz_cpu_")
@@ -117,7 +125,7 @@ def cpu_baseline(w, seconds: float, ncoeff: int, ntriplets: int) -> dict:
             rate = run(n) * scale
             return dict(value=rate / (ncoeff + 1), unit="objective+gradient evals/s", cores=cores, kind="reference",
                         objective_evals_per_s=rate,
-                        sample=f"{n} objective evaluations by the reference program (-O2 IEEE build, {cores} MPI "
+                        sample=f"{n} objective evaluations by the reference program ({flags}, {cores} MPI "
                                f"ranks, stored triplets), timed inside the program; gradient = {ncoeff} forward "
                                f"differences as its Powell forms them" + subset_note)
         except Exception as e:  # fall back to the restatement
@@ -289,6 +297,15 @@ def main():
                        f"x {ncand_launch} candidates"}
     traffic, tsrc = _traffic(dom["kernel"], w.name, ncand_launch)
     hbm_fwd = (FWD_BYTES_PER_PAIR * P + FWD_BYTES_PER_ATOM * N) * ncand_launch
+    hbm_resid = (RESID_BYTES_PER_PAIR * P + RESID_BYTES_PER_ATOM * N) * ncand_launch
+    traffic_resid, _ = _traffic("k_resid", w.name, ncand_launch)
+    peaks = None
+    if not args.no_side:
+        tf, gbs = probe_peaks()
+        peaks = {"fp64_tflops": tf, "hbm_copy_GBs": gbs,
+                 "note": "measured on this GPU: FP64 FMA loop (8 chains per lane) and a 1 GiB stream copy "
+                         "(read + write bytes); datasheet peaks are 78.6 TFLOP/s and 8000 GB/s"}
+        log(f"measured peaks: FP64 {tf:.1f} TFLOP/s, copy {gbs:.0f} GB/s")
     tile = obj.get_tile()
     out = {
         "metric": "MEAM objective+gradient evals/s, batched vs plain on 1 GPU; triplets/s; %HBM BW",
@@ -318,12 +335,20 @@ def main():
                      "frac": dom["tflops"] / PEAK_FP64_TFLOPS, "traffic": traffic,
                      "kernel": dom["kernel"], "work": dom["work"], "launch_ms": dom["ms"],
                      "traffic_source": tsrc,
+                     "frac_of_measured_peak": dom["tflops"] / peaks["fp64_tflops"] if peaks else None,
                      "note": "FP64 vector arithmetic against the published FP64 peak (78.6 TFLOP/s); "
                              "no MFMA is used"},
         "forward_kernel": {"kernel": "k_atoms", "launch_ms": fwd["ms"], "tflops": fwd["tflops"],
                            "fp64_frac": fwd["tflops"] / PEAK_FP64_TFLOPS,
                            "hbm_bytes": hbm_fwd, "hbm_GBs": hbm_fwd / (fwd["ms"] * 1e-3) / 1e9,
                            "hbm_frac": hbm_fwd / (fwd["ms"] * 1e-3) / 1e9 / PEAK_HBM_GBS},
+        # the HBM-bound kernel of the path; its time includes the final per-candidate sum
+        "force_gather_kernel": {"kernel": "k_resid", "launch_ms": ms["resid_ms"], "hbm_bytes": hbm_resid,
+                                "hbm_frac": hbm_resid / (ms["resid_ms"] * 1e-3) / 1e9 / PEAK_HBM_GBS,
+                                "traffic": traffic_resid,
+                                "traffic_frac": (traffic_resid / (ms["resid_ms"] * 1e-3) / 1e9 / PEAK_HBM_GBS
+                                                 if traffic_resid else None)},
+        "measured_peaks": peaks,
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w, args.cpu_seconds, ncoeff, T)

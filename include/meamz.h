@@ -185,6 +185,13 @@ typedef struct {
 int mz_enable_timing(mz_ctx* ctx, int on);
 int mz_last_timing(const mz_ctx* ctx, mz_timing* out);
 
+/* Peaks this GPU reaches, for roofline reports beside the datasheet figures:
+ * an FP64 FMA loop (eight independent chains per lane, TFLOP/s) and a stream
+ * copy of 1 GiB (read + write bytes, GB/s; 4 to 32 workgroups per CU), best
+ * of three timed rounds each.
+ * No reference counterpart. */
+int mz_probe_peaks(double* fp64_tflops, double* copy_gbs);
+
 /* Second derivatives y'' (and, for quintic splines, fourth derivatives y'''')
  * at the knots of one spline from its knot record [y_0..y_{n-1}, grad_L,
  * grad_R, grad2_L, grad2_R], computed on the host with the same algorithms as

@@ -75,6 +75,7 @@ def lib() -> C.CDLL:
     L.mz_last_error.restype = C.c_char_p
     L.mz_create.argtypes = [vp, vp, C.POINTER(vp)]
     L.mz_create_host_lists.argtypes = [vp, vp, C.POINTER(vp)]
+    L.mz_probe_peaks.argtypes = [dp, dp]
     L.mz_destroy.argtypes = [vp]
     L.mz_info.argtypes = [vp, C.POINTER(Info)]
     L.mz_objective.argtypes = [vp, dp, dp, dp, dp, dp, dp]
@@ -100,6 +101,14 @@ def spline_refresh(spline) -> tuple:
     _check(lib().mz_spline_refresh(int(spline.basis), int(spline.n), float(spline.x0), float(spline.xn),
                                    int(spline.bc[0] == -1), int(spline.bc[1] == -1), _dp(rec), _dp(ypp), _dp(yp4)))
     return ypp, yp4
+
+
+def probe_peaks() -> tuple:
+    """(FP64 FMA TFLOP/s, HBM copy GB/s) this GPU reaches (mz_probe_peaks)."""
+    tf = np.zeros(1)
+    gbs = np.zeros(1)
+    _check(lib().mz_probe_peaks(_dp(tf), _dp(gbs)))
+    return float(tf[0]), float(gbs[0])
 
 
 def _check(rc: int) -> None:
