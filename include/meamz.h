@@ -137,17 +137,14 @@ int mz_densities(mz_ctx* ctx, const double* knots, double* dens);
 
 /* Objective and its gradient with respect to every entry of the knot vector
  * (dF has info.nknot entries; entries of fixed knots and natural boundary
- * slots are still filled).  One reverse sweep over the same lists.  The
- * jmeam styles have no adjoint: MZ_ERR_LAYOUT (use mz_objective_batch
- * without G, or mz_residuals_batch, for their difference quotients).
+ * slots are still filled).  One reverse sweep over the same lists.
  * No reference counterpart: the reference's Powell forms forward differences,
  * src/opt_powell.cpp:348-389. */
 int mz_objective_grad(mz_ctx* ctx, const double* knots, double* F, double* dF);
 
 /* Objective (and, if G is non-NULL, gradient) of ncand knot vectors stored
  * row-major in knots[ncand*nknot], all scored over the whole database in one
- * captured launch sequence.  F[ncand], G[ncand*nknot] (G must be NULL for
- * the jmeam styles).
+ * captured launch sequence.  F[ncand], G[ncand*nknot].
  * No reference counterpart: replaces the per-member compute loops of the GA
  * src/opt_genalg.cpp:143-150,237-265 and the trial loops of simulated
  * annealing src/opt_simann.cpp:120-135 and the line searches. */

@@ -51,6 +51,7 @@ struct AdjArgs {
   long long npairs;
   const double* rho;       // [cand][natoms] unclamped densities
   const double4* G;        // [cand][npairs]: three-body leg sum G_p and S1_p = sum_k g f_k
+  const double* T1;        // jmeam: [cand][npairs][2] T1_p = sum_k q p_k per p p q term
   const double* lam;       // [cand][natoms*3]
   const double* cell_adj;  // [cand][ncells*8]
   double* slots;           // [cand][kSlots][ntab]
@@ -68,6 +69,8 @@ struct ChainArgs {
   int f0, nf, u0;
   int fper;                // one f-max penalty per f spline (gmeam)
   int nodens;              // pair styles: no U'(mid) penalty
+  int npgrp;               // max-|y| penalty groups: f, then (jmeam) p and p2
+  int pgrp0[3];            // first spline of each group (nf splines each)
   const int* nknots;
   const int* basis;
   const int* koff;
@@ -181,15 +184,56 @@ __device__ __forceinline__ GSpl gspl_of(const SplHead& H, int gco0) {
               H.coef_off - gco0, H.basis == 5};
 }
 
+// Value and d/dt of an angular spline at cs from its compact records: the
+// interval (fl, t) or, for point styles beyond the ends, the quadratic
+// extrapolation (extrap; fl 0 left, 1 right; t the offset); keep is false for
+// a knot-style cosine at or beyond x_n (1 + 1e-8), whose term is absent.
+template <bool POINT, bool Q, int NC>
+__device__ __forceinline__ void ang_at(const GSpl& G, const double* gt, double cs, double& v, double& dvt, double& t,
+                                       int& fl, bool& extrap, bool& keep) {
+  extrap = false;
+  keep = true;
+  if (POINT && (cs < G.x0 || cs > G.xn)) {
+    extrap = true;
+    const bool left = cs < G.x0;
+    const double e = cs - (left ? G.x0 : G.xn);
+    const double g1 = left ? G.gL : G.gR, g2 = left ? G.g2L : G.g2R;
+    v = (left ? G.y0 : G.yn) + g1 * e + 0.5 * g2 * e * e;
+    dvt = (g1 + g2 * e) / G.inv;
+    t = e;
+    fl = left ? 0 : 1;
+    return;
+  }
+  const double u = (cs - G.x0) * G.inv;
+  const double flo = fmin(fmax(floor(u), 0.0), G.kmax);
+  t = u - flo;
+  fl = (int)flo;
+  const double2* C = reinterpret_cast<const double2*>(gt + G.rec * NC) + fl * (NC / 2);
+  const double2 c01 = C[0], c23 = C[1];
+  if (Q) {
+    const double2 c45 = C[2];
+    v = c01.x + t * (c01.y + t * (c23.x + t * (c23.y + t * (c45.x + t * c45.y))));
+    dvt = c01.y + t * (2.0 * c23.x + t * (3.0 * c23.y + t * (4.0 * c45.x + t * 5.0 * c45.y)));
+  } else {
+    v = c01.x + t * (c01.y + t * (c23.x + t * c23.y));
+    dvt = c01.y + t * (2.0 * c23.x + t * 3.0 * c23.y);
+  }
+  if (!POINT) keep = cs < G.xcut;
+}
+
 // GR: lane replicas of the g-coefficient adjoints (32 keeps the 32 lanes that
 // one LDS cycle serves on distinct replicas; fewer when LDS is short).
 // GP: the g spline of a triplet depends on both neighbours' types (gmeam),
 // chosen per triplet from the heads staged in LDS.
-template <bool MEAM, bool POINT, bool Q, int GR, bool GP>
+// JM: jmeam styles, JM p p q terms: the partner record gains p and a p' per
+// term, the sweep a q evaluation, a push and the q-coefficient adjoints
+// (the g term with psi -> lambda_E, U' -> 1, f -> p).
+template <bool MEAM, bool POINT, bool Q, int GR, bool GP, int JM>
 __global__ void __launch_bounds__(kAdjBlock) k_adjoint(const AdjArgs A) {
   extern __shared__ __attribute__((aligned(64))) char dyn[];
-  __shared__ double pd[kPartnerFields][kAdjBlock];
-  __shared__ double push[2][kAdjBlock];
+  constexpr int NJ = JM > 0 ? JM : 1;
+  __shared__ double pd[kPartnerFields + 2 * JM][kAdjBlock];
+  __shared__ double push[2 + JM][kAdjBlock];
   __shared__ double psi[kAdjAtoms], Up[kAdjAtoms];
   static_assert(kAdjAtoms * 4 == kAdjBlock, "phase B takes four lanes per atom");
   __shared__ int poff[kAdjAtoms + 1];
@@ -238,9 +282,14 @@ __global__ void __launch_bounds__(kAdjBlock) k_adjoint(const AdjArgs A) {
   uint2 pr = make_uint2(0u, 0u);
   int ig = ch.a0;
   double G0 = 0, G1 = 0, G2 = 0, S1 = 0;
+  double T1v[NJ];
+#pragma unroll
+  for (int m = 0; m < NJ; ++m) T1v[m] = 0.0;
   if (has) {
     pr = A.pairs[ch.p0 + p];
     ig = A.owner[ch.p0 + p];
+#pragma unroll
+    for (int m = 0; m < JM; ++m) T1v[m] = A.T1[((size_t)cand * A.npairs + ch.p0 + p) * 2 + m];
     if (MEAM) {
       const double4 Gq = A.G[(size_t)cand * A.npairs + ch.p0 + p];
       G0 = Gq.x;
@@ -290,6 +339,14 @@ __global__ void __launch_bounds__(kAdjBlock) k_adjoint(const AdjArgs A) {
   double r = 0, invr = 0, ux = 0, uy = 0, uz = 0, fj = 0, dfj = 0, drho = 0, aj = 0, wx = 0, wy = 0, wz = 0;
   int rrec = -1, frec = -1;  // located rho and f interval records (knot style)
   double trho = 0, tf = 0, rinv = 0, finv = 0;
+  // jmeam: p and p' of the pair per term, and the located p interval (knot style)
+  double pvj[NJ], dpj[NJ], ptt[NJ], pinv[NJ];
+  int prec[NJ];
+#pragma unroll
+  for (int m = 0; m < NJ; ++m) {
+    pvj[m] = dpj[m] = ptt[m] = pinv[m] = 0.0;
+    prec[m] = -1;
+  }
   if (has) {
     r = __dsqrt_rn(__dadd_rn(__dadd_rn(__dmul_rn(d[0], d[0]), __dmul_rn(d[1], d[1])), __dmul_rn(d[2], d[2])));
     invr = 1.0 / r;
@@ -328,6 +385,25 @@ __global__ void __launch_bounds__(kAdjBlock) k_adjoint(const AdjArgs A) {
         }
       }
     }
+#pragma unroll
+    for (int m = 0; m < JM; ++m) {
+      const SplHead& Hp = HD[A.pot.fn_off[5 + 2 * m] + pij];
+      if (POINT) {
+        if (eval_radial<POINT, Q>(Hp, IV, r, v, dv)) {
+          pvj[m] = v;
+          dpj[m] = dv;
+        }
+      } else {
+        int k;
+        if (locate_knot(Hp, IV + Hp.coef_off, r, k, ptt[m])) {
+          prec[m] = Hp.coef_off + k;
+          pinv[m] = Hp.invstep;
+          poly<Q>(IV[prec[m]], ptt[m], pinv[m], v, dv);
+          pvj[m] = v;
+          dpj[m] = dv;
+        }
+      }
+    }
     const double vx = mxx * ux + mzx * uz, vy = myy * uy + mxy * ux, vz = mzz * uz + myz * uy;
     wx = lw[0] - r * vx;
     wy = lw[1] - r * vy;
@@ -341,6 +417,12 @@ __global__ void __launch_bounds__(kAdjBlock) k_adjoint(const AdjArgs A) {
 #pragma unroll
       for (int q = 0; q < kPartnerFields; ++q) pd[q][p] = pv[q];
       push[0][p] = 0.0;
+#pragma unroll
+      for (int m = 0; m < JM; ++m) {
+        pd[kPartnerFields + 2 * m][p] = pvj[m];
+        pd[kPartnerFields + 2 * m + 1][p] = aj * dpj[m];
+        push[2 + m][p] = 0.0;
+      }
     }
     push[1][p] = sv;  // staged for S_i; cleared again below
     if (GP) ptyp[p] = tj;
@@ -392,6 +474,9 @@ __global__ void __launch_bounds__(kAdjBlock) k_adjoint(const AdjArgs A) {
   // plus j+n/2 for the first n/2 lanes when n is even (the forward pass's
   // round-robin); the partner sums go through LDS atomics on distinct k.
   double Phi = 0.0, Phip = 0.0;
+  double Rq[NJ];  // jmeam: p-value adjoint of the lane's pair per term, own visits
+#pragma unroll
+  for (int m = 0; m < NJ; ++m) Rq[m] = 0.0;
   if (MEAM && has) {
     const double ps = psi[ia], up = Up[ia];
     const int kb = poff[ia], ke = poff[ia + 1];
@@ -402,6 +487,9 @@ __global__ void __launch_bounds__(kAdjBlock) k_adjoint(const AdjArgs A) {
     const double Aj = aj * dfj;
     const double Vjx = invr * (wx - aj * ux), Vjy = invr * (wy - aj * uy), Vjz = invr * (wz - aj * uz);
     double Rj = 0.0;
+    GSpl Gq[NJ];  // q (q2) of the atom's type
+#pragma unroll
+    for (int m = 0; m < JM; ++m) Gq[m] = gspl_of(HD[A.pot.fn_off[6 + 2 * m] + ti], A.gco0);
     int kl = l;
     for (int m = 0; m < trips; ++m) {
       kl = (kl + 1 == n) ? 0 : kl + 1;
@@ -477,6 +565,44 @@ __global__ void __launch_bounds__(kAdjBlock) k_adjoint(const AdjArgs A) {
           atomicAdd(cp + 5 * R, al * t4 * t + 5.0 * bt * t4);
         }
       }
+      // jmeam: X_q = lambda_E q + q' (B_jk + B_kj); pair j gains p_k X_q + q A_k,
+      // pair k gains p_j X_q + q A_j (A = a p'); q-coefficient adjoint of weights
+      // lambda_E p_j p_k + A_j p_k + A_k p_j (value) and p_j p_k (B_jk + B_kj) (slope)
+#pragma unroll
+      for (int m = 0; m < JM; ++m) {
+        double qv, dqt, tq;
+        int flq;
+        bool exq, kq;
+        ang_at<POINT, Q, NC>(Gq[m], gt, cs, qv, dqt, tq, flq, exq, kq);
+        if (!kq) continue;
+        const double pk = pd[kPartnerFields + 2 * m][k], Apk = pd[kPartnerFields + 2 * m + 1][k];
+        const double pj = pvj[m], Apj = aj * dpj[m];
+        const double Xq = lamE * qv + dqt * Gq[m].inv * Bs;
+        Rq[m] += pk * Xq + qv * Apk;
+        atomicAdd(&push[2 + m][k], pj * Xq + qv * Apj);
+        const double alq = lamE * pj * pk + (Apj * pk + Apk * pj);
+        const double beq = pj * pk * Bs;
+        if (POINT && exq) {
+          const bool left = flq == 0;
+          double* qhead = atab + ncoef * 6 + (A.pot.fn_off[6 + 2 * m] + ti) * 6;
+          atomicAdd(qhead + (left ? 0 : 1), alq);
+          atomicAdd(qhead + (left ? 2 : 3), alq * tq + beq);
+          if (Gq[m].quintic) atomicAdd(qhead + (left ? 4 : 5), 0.5 * alq * tq * tq + beq * tq);
+        } else {
+          const double bt = beq * Gq[m].inv;
+          const double t2 = tq * tq, t3 = t2 * tq;
+          double* cp = gacc + Gq[m].rec * NC * R + (tid & (R - 1)) + flq * (NC * R);
+          atomicAdd(cp, alq);
+          atomicAdd(cp + R, alq * tq + bt);
+          atomicAdd(cp + 2 * R, alq * t2 + 2.0 * bt * tq);
+          atomicAdd(cp + 3 * R, alq * t3 + 3.0 * bt * t2);
+          if (Q) {
+            const double t4 = t3 * tq;
+            atomicAdd(cp + 4 * R, alq * t4 + 4.0 * bt * t3);
+            atomicAdd(cp + 5 * R, alq * t4 * tq + 5.0 * bt * t4);
+          }
+        }
+      }
     }
     Phi = Rj;
   }
@@ -492,6 +618,8 @@ __global__ void __launch_bounds__(kAdjBlock) k_adjoint(const AdjArgs A) {
   if (MEAM && has) {
     Phi += push[0][p];
     Phip = Up[ia] * aj * S1;
+#pragma unroll
+    for (int m = 0; m < JM; ++m) Rq[m] += push[2 + m][p];
   }
   __syncthreads();
   if (has) {
@@ -508,6 +636,16 @@ __global__ void __launch_bounds__(kAdjBlock) k_adjoint(const AdjArgs A) {
     } else {
       if (rrec >= 0) acc_interval<Q>(sk, rrec, trho, ps, aj * up * rinv);
       if (MEAM && frec >= 0) acc_interval<Q>(sk, frec, tf, Phi, Phip * finv);
+    }
+    // jmeam p splines: value adjoint Rq, p' adjoint a T1 (no U' factor)
+#pragma unroll
+    for (int m = 0; m < JM; ++m) {
+      if (POINT) {
+        const int sp = A.pot.fn_off[5 + 2 * m] + pij;
+        acc_radial<POINT, Q>(atab, sk, ncoef, sp, HD[sp], IV, r, Rq[m], aj * T1v[m]);
+      } else if (prec[m] >= 0) {
+        acc_interval<Q>(sk, prec[m], ptt[m], Rq[m], aj * T1v[m] * pinv[m]);
+      }
     }
   }
   __syncthreads();
@@ -609,13 +747,17 @@ __global__ void __launch_bounds__(64) k_chain(const ChainArgs A) {
         const double pe = kDummyWeight * 25. * (1.0 - am) * (1.0 - am);
         rb[km] += -2.0 * pe * kDummyWeight * 25. * 2.0 * (1.0 - am) * (m > 0 ? 1.0 : -1.0);
       }
-    } else if (A.meam && s >= A.f0 && s < A.f0 + A.nf) {
-      // f-max penalty (src/pots/pot_meam_spline/pot_meam_spline.cpp:274-278):
-      // only the knot holding the signed max |y| over all f splines moves it
+    } else if (A.meam && !A.fper) {
+      // f-max penalty (src/pots/pot_meam_spline/pot_meam_spline.cpp:274-278),
+      // and the p-max ones of the jmeam styles (src/pots/pot_jmeam2/pot_jmeam2.cpp:308-324):
+      // only the knot holding the signed max |y| over the group moves its term
+      int g0 = -1;
+      for (int g = 0; g < A.npgrp; ++g)
+        if (s >= A.pgrp0[g] && s < A.pgrp0[g] + A.nf) g0 = A.pgrp0[g];
       double mx = 0.0;
       int ms = -1, mk = -1;
-      for (int q = 0; q < A.nf; ++q) {
-        const int s2 = A.f0 + q;
+      for (int q = 0; q < (g0 >= 0 ? A.nf : 0); ++q) {
+        const int s2 = g0 + q;
         const double* y = kv + A.koff[s2];
         double m = 0.0;
         int km = -1;
@@ -821,8 +963,10 @@ int setup_adjoint(mz_ctx* c) {
   // matter more than workgroups per CU.
   const int nc_g = c->pot.quintic ? 6 : 4;
   const int fixed = (X.ngco * nc_g + X.ntab) * (int)sizeof(double) + c->nspl * (int)sizeof(mz::SplHead);
-  const int stat = (int)(mz::kPartnerFields * sizeof(double) * mz::kAdjBlock + 2 * sizeof(double) * mz::kAdjBlock +
-                         2 * sizeof(double) * mz::kAdjAtoms + sizeof(int) * (mz::kAdjAtoms + 1));
+  const int jm = c->pot.njm;
+  const int stat = (int)((mz::kPartnerFields + 2 * jm) * sizeof(double) * mz::kAdjBlock +
+                         (2 + jm) * sizeof(double) * mz::kAdjBlock + 2 * sizeof(double) * mz::kAdjAtoms +
+                         sizeof(int) * (mz::kAdjAtoms + 1));
   X.grep_log2 = -1;
   for (int lg : {5, 3, 0})  // the replica counts the kernel is built for
     if (X.grep_log2 < 0 && fixed + (c->meam ? X.ngco * nc_g * (1 << lg) * 8 : 0) + stat <= 64 * 1024)
@@ -845,7 +989,8 @@ int ensure_adjoint_buffers(mz_ctx* c, int ncand) {
   const size_t nc = (size_t)ncand;
   bool bad = X.d_rho.alloc(nc * c->natoms) || X.d_dF.alloc(nc * c->natoms) || X.d_G.alloc(nc * c->npairs * 4) ||
              X.d_lam.alloc(nc * c->natoms * 3) || X.d_cadj.alloc(nc * c->ncells * 8) ||
-             X.d_slots.alloc(nc * mz::kSlots * X.ntab) || X.d_grad.alloc(nc * c->nknot);
+             X.d_slots.alloc(nc * mz::kSlots * X.ntab) || X.d_grad.alloc(nc * c->nknot) ||
+             (c->pot.njm && X.d_T1.alloc(nc * c->npairs * 2));
   if (bad) return mz_fail(MZ_ERR_NOMEM, "device allocation of gradient buffers failed");
   if (X.h_grad) (void)hipHostFree(X.h_grad);
   X.h_grad = nullptr;
@@ -858,27 +1003,29 @@ int ensure_adjoint_buffers(mz_ctx* c, int ncand) {
   return MZ_OK;
 }
 
-template <bool M, bool P, bool Q, bool GP>
+template <bool M, bool P, bool Q, bool GP, int JM>
 void launch_adjoint_g(hipStream_t st, const mz::AdjArgs& a, int nchunks, int ncand, int dyn) {
   const dim3 grid((nchunks + a.cpw - 1) / a.cpw, ncand), block(mz::kAdjBlock);
-  if (a.grep_log2 == 5) hipLaunchKernelGGL((mz::k_adjoint<M, P, Q, 32, GP>), grid, block, dyn, st, a);
-  else if (a.grep_log2 == 3) hipLaunchKernelGGL((mz::k_adjoint<M, P, Q, 8, GP>), grid, block, dyn, st, a);
-  else hipLaunchKernelGGL((mz::k_adjoint<M, P, Q, 1, GP>), grid, block, dyn, st, a);
+  if (a.grep_log2 == 5) hipLaunchKernelGGL((mz::k_adjoint<M, P, Q, 32, GP, JM>), grid, block, dyn, st, a);
+  else if (a.grep_log2 == 3) hipLaunchKernelGGL((mz::k_adjoint<M, P, Q, 8, GP, JM>), grid, block, dyn, st, a);
+  else hipLaunchKernelGGL((mz::k_adjoint<M, P, Q, 1, GP, JM>), grid, block, dyn, st, a);
 }
 
 template <bool M, bool P, bool Q>
 void launch_adjoint_t(hipStream_t st, const mz::AdjArgs& a, int nchunks, int ncand, int dyn) {
   if (!M)
-    hipLaunchKernelGGL((mz::k_adjoint<M, P, Q, 1, false>), dim3((nchunks + a.cpw - 1) / a.cpw, ncand),
+    hipLaunchKernelGGL((mz::k_adjoint<M, P, Q, 1, false, 0>), dim3((nchunks + a.cpw - 1) / a.cpw, ncand),
                        dim3(mz::kAdjBlock), dyn, st, a);
-  else if (a.pot.gpair) launch_adjoint_g<M, P, Q, true>(st, a, nchunks, ncand, dyn);
-  else launch_adjoint_g<M, P, Q, false>(st, a, nchunks, ncand, dyn);
+  else if (a.pot.gpair) launch_adjoint_g<M, P, Q, true, 0>(st, a, nchunks, ncand, dyn);
+  else if (a.pot.njm == 1) launch_adjoint_g<M, P, Q, false, 1>(st, a, nchunks, ncand, dyn);
+  else if (a.pot.njm == 2) launch_adjoint_g<M, P, Q, false, 2>(st, a, nchunks, ncand, dyn);
+  else launch_adjoint_g<M, P, Q, false, 0>(st, a, nchunks, ncand, dyn);
 }
 
 }  // namespace
 
 int mz_run_forward(mz_ctx* c, int ncand, double* d_errvec, double* d_ce, double* d_cs, double* d_rho, double* d_dF,
-                   double* d_G, double* d_lam, double* d_cadj);
+                   double* d_G, double* d_lam, double* d_cadj, double* d_T1);
 mz::AtomArgs mz_atom_args(const mz_ctx* c);
 void mz_record_timing(mz_ctx* c);
 int mz_ensure_buffers(mz_ctx* c, int ncand);
@@ -888,7 +1035,7 @@ static int run_grad(mz_ctx* c, int ncand) {
   mz::AdjointExtra& X = c->adjx;
   hipStream_t st = c->stream;
   int rc = mz_run_forward(c, ncand, nullptr, nullptr, nullptr, X.d_rho.p, X.d_dF.p, c->meam ? X.d_G.p : nullptr,
-                          X.d_lam.p, X.d_cadj.p);
+                          X.d_lam.p, X.d_cadj.p, c->pot.njm ? X.d_T1.p : nullptr);
   if (rc) return rc;
   mz::launch_zero(st, X.d_slots.p, (size_t)ncand * mz::kSlots * X.ntab);
   if (c->timing) ADJ_HIP(hipEventRecord(c->ev[5], st));
@@ -909,6 +1056,7 @@ static int run_grad(mz_ctx* c, int ncand) {
   a.npairs = c->npairs;
   a.rho = X.d_rho.p;
   a.G = reinterpret_cast<const double4*>(X.d_G.p);
+  a.T1 = X.d_T1.p;
   a.lam = X.d_lam.p;
   a.cell_adj = X.d_cadj.p;
   a.slots = X.d_slots.p;
@@ -950,6 +1098,8 @@ static int run_grad(mz_ctx* c, int ncand) {
   ca.fper = c->meam && c->nfpen == ca.nf && (c->style == mz::kGMeamSpline || c->style == mz::kGMeam);
   ca.u0 = c->pot.fn_off[2];
   ca.nodens = c->pot.nodens;
+  ca.npgrp = c->meam && !ca.fper ? 1 + c->pot.njm : 0;
+  for (int g = 0; g < 3; ++g) ca.pgrp0[g] = c->pot.fn_off[g == 0 ? 3 : 3 + 2 * g];
   ca.nknots = c->d_nknots.p;
   ca.basis = c->d_basis.p;
   ca.koff = c->d_koff.p;
@@ -970,12 +1120,11 @@ static int run_grad(mz_ctx* c, int ncand) {
 
 int mz_grad_impl(mz_ctx* c, int ncand, const double* knots, double* F, double* G) {
   if (!c || !knots || !F || !G || ncand < 1) return mz_fail(MZ_ERR_ARG, "null argument or ncand < 1");
-  if (c->pot.njm) return mz_fail(MZ_ERR_LAYOUT, "no adjoint gradient for the jmeam styles (objective and residuals only)");
   int rc = setup_adjoint(c);
   if (rc) return rc;
   mz::AdjointExtra& X = c->adjx;
   // bound the memory of the stored leg vectors: sub-batches of candidates
-  const size_t per_cand = (size_t)c->npairs * 7 * sizeof(double) + (size_t)c->natoms * 120 + (size_t)c->ncells * 64 +
+  const size_t per_cand = (size_t)c->npairs * (c->pot.njm ? 9 : 7) * sizeof(double) + (size_t)c->natoms * 120 + (size_t)c->ncells * 64 +
                           c->chunks.size() * 64 + (size_t)mz::kSlots * X.ntab * 8;
   int sub = (int)std::max<size_t>(1, std::min<size_t>((size_t)ncand, (size_t)(48ull << 30) / std::max<size_t>(per_cand, 1)));
   mz_timing acc{};  // kernel times summed over the sub-batches of a timed call
@@ -1044,7 +1193,7 @@ void mz_adjoint_drop(mz_ctx* c) {
   for (auto& g : X.graphs)
     if (g.ex) (void)hipGraphExecDestroy(g.ex);
   X.graphs.clear();
-  X.d_rho.release(); X.d_dF.release(); X.d_G.release(); X.d_lam.release(); X.d_cadj.release();
+  X.d_rho.release(); X.d_dF.release(); X.d_G.release(); X.d_lam.release(); X.d_cadj.release(); X.d_T1.release();
   X.d_slots.release(); X.d_grad.release(); X.d_J.release(); X.d_joff.release(); X.d_chunks.release();
   X.d_owner.release();
   if (X.h_grad) (void)hipHostFree(X.h_grad);

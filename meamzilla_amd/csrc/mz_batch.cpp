@@ -10,7 +10,7 @@
 #include "mz_adjoint.h"
 
 int mz_run_forward(mz_ctx* c, int ncand, double* d_errvec, double* d_ce, double* d_cs, double* d_rho, double* d_dF,
-                   double* d_G, double* d_lam, double* d_cadj);
+                   double* d_G, double* d_lam, double* d_cadj, double* d_T1);
 int mz_stage_knots(mz_ctx* c, int ncand, const double* knots);
 int mz_batch_impl(mz_ctx* c, int ncand, const double* knots, double* F, double* G);
 int mz_residuals_batch_impl(mz_ctx* c, int ncand, const double* knots, double* F, double* errvec);

@@ -68,6 +68,7 @@ struct AdjointExtra {
   DevBuf<double> d_J;                     // refresh Jacobians
   DevBuf<int> d_joff;
   DevBuf<double> d_rho, d_dF, d_G, d_lam, d_cadj, d_slots, d_grad;
+  DevBuf<double> d_T1;  // jmeam: [cap][npairs][2] sum_k q p_k of every pair
   double* h_grad = nullptr;               // pinned [cap][nknot]
   struct Graph {
     hipGraphExec_t ex = nullptr;

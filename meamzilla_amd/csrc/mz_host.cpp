@@ -94,10 +94,22 @@ int build_layout(mz_ctx* c, const mz_layout* L) {
     if (c->basis[s] == 5) quintic = 1;
     c->koff[s] = koff;
     koff += c->nknots[s] + 4;
-    c->coef_off[s] = coff;
-    coff += c->nknots[s] - 1;
     c->grids[s].init(c->nknots[s], c->x0[s], c->xn[s]);
   }
+  // interval records: radial and embedding splines first, then the angular
+  // ones (g, and q, q2 of the jmeam styles), which the kernels stage and
+  // accumulate as one contiguous range
+  auto angular = [&](int s) {
+    for (int f : {4, 6, 8})
+      if (f < nfn && s >= c->pot.fn_off[f] && s < c->pot.fn_off[f] + counts[f]) return true;
+    return false;
+  };
+  for (int pass = 0; pass < 2; ++pass)
+    for (int s = 0; s < tot; ++s)
+      if (angular(s) == (pass == 1)) {
+        c->coef_off[s] = coff;
+        coff += c->nknots[s] - 1;
+      }
   c->nknot = koff;
   c->ncoef = coff;
   c->table_bytes = (int)(sizeof(mz::SplHead) * tot + sizeof(mz::SplInterval) * coff);
@@ -509,7 +521,7 @@ mz::FinalArgs final_args(const mz_ctx* c) {
 // Launch sequence of one batch of ncand evaluations whose knot vectors are
 // already in d_knots.  Shared by the plain, gradient and batched entry points.
 int mz_run_forward(mz_ctx* c, int ncand, double* d_errvec, double* d_ce, double* d_cs, double* d_rho, double* d_dF,
-                   double* d_G, double* d_lam, double* d_cadj) {
+                   double* d_G, double* d_lam, double* d_cadj, double* d_T1) {
   hipStream_t st = c->stream;
   if (c->timing) MZ_HIP(hipEventRecord(c->ev[0], st));
   mz::launch_refresh(st, c->d_knots.p, c->nknot, refresh_meta(c), c->d_tables.p, c->table_bytes, ncand);
@@ -519,6 +531,7 @@ int mz_run_forward(mz_ctx* c, int ncand, double* d_errvec, double* d_ce, double*
   aa.rho_out = d_rho;
   aa.dF_out = d_dF;
   aa.G_out = d_G;
+  aa.T1_out = d_T1;
   aa.ncand = ncand;
   const int ppl = std::max(1, c->tile.max_pairs / c->tile.block);
   // candidates per workgroup: the tile's choice, at most what ncand fills and
@@ -585,7 +598,7 @@ int mz_forward_graph(mz_ctx* c, int ncand, const double* knots, double* F) {
     const bool timing = c->timing;
     c->timing = false;
     hipError_t e1 = hipMemcpyAsync(c->d_knots.p, c->h_stage, n * sizeof(double), hipMemcpyHostToDevice, c->stream);
-    int rc = mz_run_forward(c, ncand, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
+    int rc = mz_run_forward(c, ncand, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
     hipError_t e2 = hipMemcpyAsync(c->h_F, c->d_F.p, ncand * sizeof(double), hipMemcpyDeviceToHost, c->stream);
     c->timing = timing;
     hipError_t e3 = hipStreamEndCapture(c->stream, &graph);
@@ -850,7 +863,7 @@ int mz_objective(mz_ctx* c, const double* knots, double* F, double* errvec, doub
   rc = mz_stage_knots(c, 1, knots);
   if (rc) return rc;
   rc = mz_run_forward(c, 1, errvec ? c->d_errvec.p : nullptr, cell_energy ? c->d_cell_e.p : nullptr,
-                      cell_stress ? c->d_cell_s.p : nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
+                      cell_stress ? c->d_cell_s.p : nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
   if (rc) return rc;
   hipStream_t st = c->stream;
   double Fh = 0.0;
@@ -876,7 +889,7 @@ int mz_densities(mz_ctx* c, const double* knots, double* dens) {
   if (c->d_rho.alloc(c->natoms)) return fail(MZ_ERR_NOMEM, "allocation failed");
   rc = mz_stage_knots(c, 1, knots);
   if (rc) return rc;
-  rc = mz_run_forward(c, 1, nullptr, nullptr, nullptr, c->d_rho.p, nullptr, nullptr, nullptr, nullptr);
+  rc = mz_run_forward(c, 1, nullptr, nullptr, nullptr, c->d_rho.p, nullptr, nullptr, nullptr, nullptr, nullptr);
   if (rc) return rc;
   MZ_HIP(hipMemcpyAsync(dens, c->d_rho.p, sizeof(double) * c->natoms, hipMemcpyDeviceToHost, c->stream));
   MZ_HIP(hipStreamSynchronize(c->stream));
@@ -952,7 +965,7 @@ int mz_batch_impl(mz_ctx* c, int ncand, const double* knots, double* F, double* 
   if (sub < ncand) return fail(MZ_ERR_CAPACITY, "timed batch larger than one sub-batch");
   rc = mz_stage_knots(c, ncand, knots);
   if (rc) return rc;
-  rc = mz_run_forward(c, ncand, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
+  rc = mz_run_forward(c, ncand, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
   if (rc) return rc;
   MZ_HIP(hipMemcpyAsync(F, c->d_F.p, sizeof(double) * ncand, hipMemcpyDeviceToHost, c->stream));
   MZ_HIP(hipStreamSynchronize(c->stream));
@@ -976,7 +989,7 @@ int mz_residuals_batch_impl(mz_ctx* c, int ncand, const double* knots, double* F
     const int nb = std::min(sub, ncand - c0);
     rc = mz_stage_knots(c, nb, knots + (size_t)c0 * c->nknot);
     if (rc) return rc;
-    rc = mz_run_forward(c, nb, c->d_errvec.p, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
+    rc = mz_run_forward(c, nb, c->d_errvec.p, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
     if (rc) return rc;
     MZ_HIP(hipMemcpyAsync(F + c0, c->d_F.p, sizeof(double) * nb, hipMemcpyDeviceToHost, st));
     MZ_HIP(hipMemcpyAsync(errvec + (size_t)c0 * c->nerr, c->d_errvec.p, sizeof(double) * nb * c->nerr,

@@ -42,6 +42,7 @@ struct AtomArgs {
   double* dF_out;         // optional [ncand][natoms] U'(rho)
   double* G_out;          // optional [ncand][npairs][4]: dF-free three-body leg vector G_p and
                           // S1_p = sum_k g f_k (the f'-adjoint factor of the reverse sweep)
+  double* T1_out;         // optional, jmeam: [ncand][npairs][2] T1_p = sum_k q f_k (p-term p'-adjoint factor)
 };
 
 struct ResidArgs {

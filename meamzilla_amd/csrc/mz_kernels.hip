@@ -583,6 +583,7 @@ __global__ void __launch_bounds__(BLOCK) k_atoms(const AtomArgs args) {
         // seen from both legs) and their legs bypass U'
 #pragma unroll
         for (int m = 0; m < JM; ++m) {
+          if (args.T1_out && valid[c]) args.T1_out[((size_t)cand[c] * args.npairs + ch.p0 + p) * 2 + m] = T[m][c][0];
           e_acc[c] += 0.5 * pv[m][s][c] * T[m][c][0];
           const double pr = pv[m][s][c] * invr[s];
           const double rq = dpv[m][s][c] * T[m][c][0] - pr * T[m][c][1];

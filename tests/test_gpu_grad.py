@@ -8,8 +8,7 @@ from conftest import golden_cases, load_case
 
 import pyoracle
 from meamzilla_amd import gen
-from meamzilla_amd import formats
-from meamzilla_amd._lib import MeamzError, Objective
+from meamzilla_amd._lib import Objective
 
 pytestmark = pytest.mark.gpu
 GTOL = 1e-6
@@ -48,13 +47,6 @@ def test_adjoint_matches_gpu_central_differences(case):
     pot, db, exp = load_case(case)
     obj = Objective(pot, db)
     kv = pot.knot_vector()
-    if formats.jmeam_terms(pot.style):
-        # no adjoint for the jmeam styles: the call refuses, nothing runs
-        with pytest.raises(MeamzError):
-            obj.objective_grad(kv)
-        with pytest.raises(MeamzError):
-            obj.batch(np.stack([kv, kv]), grad=True)
-        return
     F, g = obj.objective_grad(kv)
     assert abs(F - exp["F"]) <= 1e-10 * abs(exp["F"])
     idx = np.arange(len(kv))
@@ -140,6 +132,34 @@ def test_gmeam_ternary_matches_oracle_and_central_differences():
     slots = np.asarray(pot.coeff_slots())
     idx = np.sort(np.random.default_rng(5).choice(slots, size=40, replace=False))
     gcd = _cd(obj.objective, kv, idx)
+    assert _rel(g[idx], gcd) < GTOL, _rel(g[idx], gcd)
+
+
+@pytest.mark.parametrize("style", ["jmeam2/spline", "jmeam2"])
+def test_jmeam2_ternary_adjoint_matches_oracle_central_differences(style):
+    """jmeam2 with three elements (two p p q terms beside g): the adjoint
+    against central differences of the oracle's objective on a seeded sample
+    of coefficients (p, q, p2, q2 included), batched against single."""
+    _say("jmeam2 ternary", style)
+    rng = np.random.Generator(np.random.PCG64(17))
+    w = gen.workload(2, scale=8 / 512)
+    db = w.db.subset([c for c in range(w.db.ncells) if w.db.natoms[c] <= 200][:3])
+    db.types = rng.integers(0, 3, size=db.total_atoms).astype(np.int32)
+    nk = dict(phi=10, rho=10, U=10, f=10, g=9, p=9, q=8)
+    pot = gen.make_potential(rng, style, 3, 5.0, nk, (-1.0, 3.0))
+    truth = gen.make_potential(rng, style, 3, 5.0, nk, (-1.0, 3.0))
+    e, s, f = pyoracle.evaluator(truth, db)
+    db.energy0, db.stress0, db.force0 = e, s, f
+    o = pyoracle.Oracle(pot, db)
+    obj = Objective(pot, db)
+    kv = pot.knot_vector()
+    F, g = obj.objective_grad(kv)
+    assert abs(F - o.eval()["F"]) <= 1e-10 * abs(F)
+    Fb, Gb = obj.batch(np.stack([kv, kv]), grad=True)
+    assert np.allclose(Fb, F, rtol=1e-12) and _rel(Gb[1], g) < 1e-12
+    slots = np.asarray(pot.coeff_slots())
+    idx = np.sort(np.random.default_rng(7).choice(slots, size=48, replace=False))
+    gcd = _cd(o.objective, kv, idx)
     assert _rel(g[idx], gcd) < GTOL, _rel(g[idx], gcd)
 
 

@@ -42,7 +42,7 @@ def _same(pot, db, grad=True):
 def test_device_lists_match_host_lists_golden(case):
     _say("lists golden", case)
     pot, db, _ = load_case(case)
-    _same(pot, db, grad="jmeam" not in pot.style)
+    _same(pot, db)
 
 
 @pytest.mark.parametrize("cfg,scale", [(0, 1.0), (1, 24 / 256), (2, 40 / 512), (3, 30 / 1000), (4, 3 / 2000)])
