@@ -163,6 +163,24 @@ def test_jmeam2_ternary_adjoint_matches_oracle_central_differences(style):
     assert _rel(g[idx], gcd) < GTOL, _rel(g[idx], gcd)
 
 
+@pytest.mark.parametrize("case", ["eam_spline_c0", "meam_spline_binary", "jmeam_spline_binary"])
+def test_torch_gradcheck(case):
+    """torch.autograd.gradcheck of the objective through the thin
+    torch.autograd.Function over mz_objective_grad: torch's own central
+    differences (eps 1e-6) against the adjoint, element by element."""
+    import torch
+    from meamzilla_amd.autograd import objective
+    _say("gradcheck", case)
+    pot, db, _ = load_case(case)
+    obj = Objective(pot, db)
+    kv = torch.tensor(pot.knot_vector(), dtype=torch.float64, requires_grad=True)
+    _, g = obj.objective_grad(pot.knot_vector())
+    # absolute floor: the difference quotient of F (~1e4) at eps 1e-6 carries
+    # ~1e-5 of rounding, and a 1e-7 share of the largest entry
+    atol = max(1e-4, 1e-7 * float(np.max(np.abs(g))))
+    assert torch.autograd.gradcheck(lambda k: objective(k, obj), (kv,), eps=1e-6, atol=atol, rtol=1e-5)
+
+
 def test_batched_gradient_matches_single():
     _say("batched gradient")
     w = gen.workload(2, scale=12 / 512)
