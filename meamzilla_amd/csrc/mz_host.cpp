@@ -305,12 +305,17 @@ long long count_triplets(const mz_ctx* c, int gi, const double* pos, const doubl
     // a point-style triplet is kept iff both legs lie inside f
     return m * (m - 1) / 2;
   }
-  const Grid& g = c->grids[c->pot.fn_off[4] + ti];
+  // g of the triplet: by the type of i (meam), or by i and the types of j, k
+  // (gmeam, src/pot_fns.cpp:154-171)
+  const int npij = nt * (nt + 1) / 2;
   const long long n = e - b;
-  if (g.xcut() > 1.0 + 1e-12) return n * (n - 1) / 2;  // |cos| <= 1: every triplet lies inside g
+  if (c->pot.g_keep_all) return n * (n - 1) / 2;  // |cos| <= 1: every triplet lies inside every g
+  if (!c->pot.gpair && c->grids[c->pot.fn_off[4] + ti].xcut() > 1.0 + 1e-12) return n * (n - 1) / 2;
   std::vector<double> u(3 * (size_t)n);
+  std::vector<int> tk((size_t)n);
   for (long long p = b; p < e; ++p) {
     const int j = (int)c->pairs[p].x;
+    tk[p - b] = c->types[j];
     const uint32_t im = c->pairs[p].y;
     const int ix = (int8_t)(im & 0xff), iy = (int8_t)((im >> 8) & 0xff), iz = (int8_t)((im >> 16) & 0xff);
     const double* pj = pos + 3 * (size_t)j;
@@ -321,11 +326,11 @@ long long count_triplets(const mz_ctx* c, int gi, const double* pos, const doubl
     for (int q = 0; q < 3; ++q) u[3 * (p - b) + q] = d[q] / r;
   }
   long long cnt = 0;
-  const double xc = g.xcut();
   for (long long p = 0; p < n - 1; ++p)
     for (long long q = p + 1; q < n; ++q) {
+      const int gs = c->pot.gpair ? ti * npij + pair_ij_h(tk[p], tk[q], nt) : ti;
       const double cs = u[3 * p] * u[3 * q] + u[3 * p + 1] * u[3 * q + 1] + u[3 * p + 2] * u[3 * q + 2];
-      if (cs < xc) ++cnt;
+      if (cs < c->grids[c->pot.fn_off[4] + gs].xcut()) ++cnt;
     }
   return cnt;
 }
@@ -686,9 +691,13 @@ static int create_impl(const mz_layout* layout, const mz_db* db, bool device_lis
     return fail(MZ_ERR_HIP, "no HIP device available");
   }
   const auto t_list = std::chrono::steady_clock::now();
+  // the triplet counts read the list on the host unless every triplet of an
+  // atom is kept (knot-style g spanning |cos| <= 1)
+  const bool host_pairs = c->meam && (c->point || !c->pot.g_keep_all || c->pot.njm > 0);
   if (device_lists) {
     std::string e;
-    rc = mz::build_pairs_device(c, db->pos, db->box, max_rcut, e);
+    rc = mz::build_pairs_device(c, db->pos, db->box, max_rcut, host_pairs, e);
+    if (!rc) rc = mz::build_incoming_device(c, e);
     if (rc) {
       mz_destroy(c);
       return fail(rc, e);
@@ -752,12 +761,13 @@ static int create_impl(const mz_layout* layout, const mz_db* db, bool device_lis
   c->ntriplets = 0;
   for (int a = 0; a < na; ++a) c->ntriplets += c->ntrip_atom[a];
   // incoming pairs of every atom (counting sort of the pairs by neighbour):
-  // the force on atom j from pair (i -> j) is -T of that pair
-  c->in_off.assign(na + 1, 0);
-  for (long long q = 0; q < np; ++q) c->in_off[c->pairs[q].x + 1]++;
-  for (int a = 0; a < na; ++a) c->in_off[a + 1] += c->in_off[a];
-  c->in_list.assign(np, 0);
-  {
+  // the force on atom j from pair (i -> j) is -T of that pair; the device
+  // builder made them already
+  if (!device_lists) {
+    c->in_off.assign(na + 1, 0);
+    for (long long q = 0; q < np; ++q) c->in_off[c->pairs[q].x + 1]++;
+    for (int a = 0; a < na; ++a) c->in_off[a + 1] += c->in_off[a];
+    c->in_list.assign(np, 0);
     std::vector<int> fill(c->in_off.begin(), c->in_off.end() - 1);
     for (long long q = 0; q < np; ++q) c->in_list[fill[c->pairs[q].x]++] = (int)q;
   }
@@ -767,9 +777,9 @@ static int create_impl(const mz_layout* layout, const mz_db* db, bool device_lis
   c->stress_weight = db->stress_weight;
   std::vector<double> s0(db->stress0, db->stress0 + 6 * (size_t)c->ncells);
   bool bad = (!device_lists && (c->d_atom_poff.upload(c->atom_poff.data(), c->atom_poff.size()) ||
-                                 c->d_pairs.upload(c->pairs.data(), c->pairs.size()))) ||
-             c->d_in_off.upload(c->in_off.data(), c->in_off.size()) ||
-             c->d_in_list.upload(c->in_list.data(), c->in_list.size()) ||
+                                 c->d_pairs.upload(c->pairs.data(), c->pairs.size()) ||
+                                 c->d_in_off.upload(c->in_off.data(), c->in_off.size()) ||
+                                 c->d_in_list.upload(c->in_list.data(), c->in_list.size()))) ||
              c->d_pos.upload(db->pos, 3 * (size_t)na) || c->d_box.upload(db->box, 9 * (size_t)c->ncells) ||
              c->d_types.upload(c->types.data(), na) || c->d_cell_of.upload(c->cell_of.data(), na) ||
              c->d_natoms_c.upload(c->natoms_c.data(), c->ncells) || c->d_abase.upload(c->abase.data(), c->ncells) ||

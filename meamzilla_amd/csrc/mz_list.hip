@@ -129,6 +129,37 @@ __global__ void __launch_bounds__(128) k_list(const ListArgs a) {
   if (!FILL) a.counts[gi] = cnt;
 }
 
+// Incoming-pair index (the pairs whose neighbour is each atom, ascending pair
+// index within an atom, as the host's counting sort leaves them): count per
+// neighbour, fill in any order, then sort each atom's short run.
+__global__ void __launch_bounds__(256) k_in_count(const uint2* pairs, long long np, int* cnt) {
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < np) atomicAdd(cnt + pairs[q].x, 1);
+}
+
+__global__ void __launch_bounds__(256) k_in_fill(const uint2* pairs, long long np, const int* off, int* fill,
+                                                 int* list) {
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= np) return;
+  const int j = (int)pairs[q].x;
+  list[off[j] + atomicAdd(fill + j, 1)] = (int)q;
+}
+
+__global__ void __launch_bounds__(256) k_in_sort(const int* off, int* list, int na) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= na) return;
+  const int b = off[a], e = off[a + 1];
+  for (int i = b + 1; i < e; ++i) {
+    const int v = list[i];
+    int k = i - 1;
+    while (k >= b && list[k] > v) {
+      list[k + 1] = list[k];
+      --k;
+    }
+    list[k + 1] = v;
+  }
+}
+
 #define MZ_LCHECK(x)                                                                    \
   do {                                                                                  \
     hipError_t e_ = (x);                                                                \
@@ -140,7 +171,40 @@ __global__ void __launch_bounds__(128) k_list(const ListArgs a) {
 
 }  // namespace
 
-int build_pairs_device(mz_ctx* c, const double* pos, const double* box, double max_rcut, std::string& err) {
+int build_incoming_device(mz_ctx* c, std::string& err) {
+  const int na = c->natoms;
+  const long long np = c->npairs;
+  hipStream_t st = c->stream;
+  struct Scratch {
+    DevBuf<int> cnt;
+    ~Scratch() { cnt.release(); }
+  } tmp;
+  if (tmp.cnt.alloc(na) || c->d_in_off.alloc((size_t)na + 1) || c->d_in_list.alloc((size_t)std::max(np, 1LL))) {
+    err = "incoming-pair index: device allocation failed";
+    return MZ_ERR_NOMEM;
+  }
+  MZ_LCHECK(hipMemsetAsync(tmp.cnt.p, 0, sizeof(int) * (size_t)na, st));
+  const unsigned nbq = (unsigned)((np + 255) / 256);
+  if (np) hipLaunchKernelGGL(k_in_count, dim3(nbq), dim3(256), 0, st, c->d_pairs.p, np, tmp.cnt.p);
+  MZ_LCHECK(hipGetLastError());
+  std::vector<int> cnt(na);
+  MZ_LCHECK(hipMemcpyAsync(cnt.data(), tmp.cnt.p, sizeof(int) * (size_t)na, hipMemcpyDeviceToHost, st));
+  MZ_LCHECK(hipStreamSynchronize(st));
+  c->in_off.assign((size_t)na + 1, 0);
+  for (int a = 0; a < na; ++a) c->in_off[a + 1] = c->in_off[a] + cnt[a];
+  MZ_LCHECK(hipMemcpyAsync(c->d_in_off.p, c->in_off.data(), sizeof(int) * ((size_t)na + 1), hipMemcpyHostToDevice, st));
+  MZ_LCHECK(hipMemsetAsync(tmp.cnt.p, 0, sizeof(int) * (size_t)na, st));
+  if (np) hipLaunchKernelGGL(k_in_fill, dim3(nbq), dim3(256), 0, st, c->d_pairs.p, np, c->d_in_off.p, tmp.cnt.p,
+                             c->d_in_list.p);
+  hipLaunchKernelGGL(k_in_sort, dim3((unsigned)((na + 255) / 256)), dim3(256), 0, st, c->d_in_off.p, c->d_in_list.p,
+                     na);
+  MZ_LCHECK(hipGetLastError());
+  MZ_LCHECK(hipStreamSynchronize(st));
+  return MZ_OK;
+}
+
+int build_pairs_device(mz_ctx* c, const double* pos, const double* box, double max_rcut, bool host_copy,
+                       std::string& err) {
   const int na = c->natoms, nc = c->ncells;
   hipStream_t st = c->stream;
   // per-cell image ranges and fractional coordinates, as the host builder forms them
@@ -288,9 +352,12 @@ int build_pairs_device(mz_ctx* c, const double* pos, const double* box, double m
   a.pairs = c->d_pairs.p;
   launch(true);
   MZ_LCHECK(hipGetLastError());
-  // the host keeps a copy for the triplet counts and the incoming-pair index
-  c->pairs.resize(np);
-  if (np) MZ_LCHECK(hipMemcpyAsync(c->pairs.data(), c->d_pairs.p, sizeof(uint2) * (size_t)np, hipMemcpyDeviceToHost, st));
+  // a host copy only where the triplet counts need the geometry
+  if (host_copy) {
+    c->pairs.resize(np);
+    if (np)
+      MZ_LCHECK(hipMemcpyAsync(c->pairs.data(), c->d_pairs.p, sizeof(uint2) * (size_t)np, hipMemcpyDeviceToHost, st));
+  }
   MZ_LCHECK(hipStreamSynchronize(st));
   return MZ_OK;
 }
