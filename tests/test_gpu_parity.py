@@ -36,6 +36,27 @@ def test_gpu_matches_reference_golden(case):
         assert rel_err(obj.densities(), exp["densities"]) < TOL
 
 
+def test_gpu_gmeam_cut_g_counts_match_oracle():
+    """gmeam/spline with one g spline (type A, types A B) ending at cos = 0.5:
+    triplets beyond it drop out, counted per triplet from the neighbours'
+    types; counts and objective against the oracle."""
+    _say("gmeam cut g")
+    w = gen.workload(2, scale=16 / 512)
+    rng = np.random.Generator(np.random.PCG64(23))
+    nk = dict(phi=10, rho=10, U=10, f=10, g=10)
+    w.pot = gen.make_potential(rng, "gmeam/spline", 2, 5.0, nk, (-1.0, 3.0))
+    w.truth = gen.make_potential(rng, "gmeam/spline", 2, 5.0, nk, (-1.0, 3.0))
+    g0 = sum(formats.function_counts("gmeam/spline", 2)[:4])
+    w.pot.splines[g0 + 1].xn = 0.5
+    gen.add_targets(w, pyoracle.evaluator)
+    o = pyoracle.Oracle(w.pot, w.db)
+    obj = Objective(w.pot, w.db)
+    assert (obj.info.npairs, obj.info.ntriplets) == o.counts()
+    e, r = o.eval(), obj.compute()
+    assert abs(r["F"] - e["F"]) <= TOL * abs(e["F"])
+    assert rel_err(r["errvec"], e["errvec"]) < TOL
+
+
 @pytest.mark.parametrize("style", ["jmeam/spline", "jmeam", "jmeam2/spline", "jmeam2"])
 def test_gpu_jmeam_styles_match_oracle(style):
     """jmeam styles on the binary workload slice (a few thousand atoms): the
