@@ -181,6 +181,17 @@ def _traffic(kernel: str, workload: str, cand: int):
         return None, None
 
 
+def _fp64_ops(kernel: str, workload: str, cand: int):
+    """Executed FP64 operations per launch of `kernel` from the committed
+    rocprofv3 VALU-counter summary, if one exists for this workload."""
+    path = os.path.join(ROOT, "profiles", "fp64.json")
+    try:
+        with open(path) as fh:
+            return json.load(fh)[f"{kernel}|{workload}|{cand}"]["fp64_ops_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def _num(x: float):
     return None if x != x else x
 
@@ -299,6 +310,7 @@ def main():
     hbm_fwd = (FWD_BYTES_PER_PAIR * P + FWD_BYTES_PER_ATOM * N) * ncand_launch
     hbm_resid = (RESID_BYTES_PER_PAIR * P + RESID_BYTES_PER_ATOM * N) * ncand_launch
     traffic_resid, _ = _traffic("k_resid", w.name, ncand_launch)
+    ex_ops = _fp64_ops(dom["kernel"], w.name, ncand_launch)
     peaks = None
     if not args.no_side:
         tf, gbs = probe_peaks()
@@ -336,6 +348,8 @@ def main():
                      "kernel": dom["kernel"], "work": dom["work"], "launch_ms": dom["ms"],
                      "traffic_source": tsrc,
                      "frac_of_measured_peak": dom["tflops"] / peaks["fp64_tflops"] if peaks else None,
+                     "executed_tflops": (ex_ops / (dom["ms"] * 1e-3) / 1e12 if ex_ops else None),
+                     "executed_source": "profiles/fp64.json (rocprofv3 SQ_INSTS_VALU_*_F64)" if ex_ops else None,
                      "note": "FP64 vector arithmetic against the published FP64 peak (78.6 TFLOP/s); "
                              "no MFMA is used"},
         "forward_kernel": {"kernel": "k_atoms", "launch_ms": fwd["ms"], "tflops": fwd["tflops"],
