@@ -77,6 +77,16 @@ def test_driver_genalg_runs_batched(tmp_path):
 
 
 @pytest.mark.gpu
+def test_driver_simann_keeps_the_best(tmp_path):
+    """One temperature of simulated annealing (3 ncoeff x 20 x ncoeff Gaussian-bump
+    trials, src/opt_simann.cpp:89-206): the returned potential is the best seen,
+    so its objective is at most the start's."""
+    ini, fin, out, d = _run_fit(tmp_path, "eam_spline_c0", "simann", ["max_steps 1", "seed 3", "anneal_temp 1.0"])
+    assert "T=" in out and fin[0] <= ini[0]
+    assert os.path.exists(d / "pot.end")
+
+
+@pytest.mark.gpu
 def test_fitrun_lbfgs_and_powell(tmp_path):
     """End-to-end fit run: files in the reference's formats, the C++ driver,
     L-BFGS on the adjoint gradient and Powell on the same objective."""
