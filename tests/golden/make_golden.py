@@ -266,6 +266,19 @@ def jmeam_cases():
     pot2.splines[17].xn = 4.4         # p2_AA ends early
     _finish("jmeam2_point_binary", pot2, _slice(w, small[:2]), truth=truth2,
             note="binary jmeam2 (point style), p2_AA on [1.8,4.4]")
+    # quintic (spline5eq) splines throughout, one element
+    pot5 = gen.make_potential(rng, "jmeam2/spline", 1, 5.0, nk, (-1.0, 3.0), basis=5)
+    truth5 = gen.make_potential(rng, "jmeam2/spline", 1, 5.0, nk, (-1.0, 3.0), basis=5)
+    db5 = _slice(w, small[:2])
+    db5.types[:] = 0
+    _finish("jmeam2_spline_q5", pot5, db5, truth=truth5,
+            note="one-element jmeam2/spline, spline5eq throughout")
+    pot5 = pot5.copy()
+    pot5.style = "jmeam2"
+    db5 = _slice(w, small[2:4])
+    db5.types[:] = 0
+    _finish("jmeam2_point_q5", pot5, db5, truth=truth5,
+            note="one-element jmeam2 (point style), spline5eq throughout")
 
 
 if __name__ == "__main__":
