@@ -210,6 +210,11 @@ def gmeam_cases():
     pot.splines[8].xn = 4.2          # one f spline (f_AB) ends before the phi cutoff
     _finish("gmeam_point_binary", pot, _slice(w, small[:3]), truth=truth,
             note="binary gmeam (point style), f_AB on [1.8,4.2]")
+    # quintic (spline5eq) splines throughout
+    pot5 = gen.make_potential(rng, "gmeam/spline", 2, 5.0, nk, (-1.0, 3.0), basis=5)
+    truth5 = gen.make_potential(rng, "gmeam/spline", 2, 5.0, nk, (-1.0, 3.0), basis=5)
+    _finish("gmeam_spline_q5", pot5, _slice(w, small[2:5]), truth=truth5,
+            note="binary gmeam/spline, spline5eq throughout")
 
 
 def pair_cases():
