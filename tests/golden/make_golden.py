@@ -190,6 +190,7 @@ def main():
     gmeam_cases()
     pair_cases()
     jmeam_cases()
+    point_quintic_cases()
 
 
 def gmeam_cases():
@@ -286,11 +287,31 @@ def jmeam_cases():
             note="one-element jmeam2 (point style), spline5eq throughout")
 
 
+def point_quintic_cases():
+    """Point-evaluated eam and meam with spline5eq splines (quadratic end
+    extrapolation from grad2).  ``python tests/golden/make_golden.py pointq5``
+    writes only these."""
+    rng = np.random.Generator(np.random.PCG64(9090))
+    w = gen.workload(1, scale=4 / 256, basis=5)
+    pot = w.pot.copy()
+    pot.style = "meam"
+    pot.splines[3].xn = 4.3          # f ends before phi: triplets with a leg beyond it drop out
+    _finish("meam_point_q5", pot, _slice(w, [0, 2]), truth=w.truth,
+            note="meam (point style), spline5eq throughout, f on [1.8,4.3]")
+    w0 = gen.workload(0, scale=4 / 16)
+    nk = dict(phi=10, rho=10, U=8)
+    pot0 = gen.make_potential(rng, "eam", 1, 5.5, nk, (0.0, 2.0), basis=5)
+    truth0 = gen.make_potential(rng, "eam", 1, 5.5, nk, (0.0, 2.0), basis=5)
+    _finish("eam_point_q5", pot0, w0.db, truth=truth0, note="eam (point style), spline5eq throughout")
+
+
 if __name__ == "__main__":
     if len(sys.argv) > 1 and sys.argv[1] == "gmeam":
         gmeam_cases()
     elif len(sys.argv) > 1 and sys.argv[1] == "jmeam":
         jmeam_cases()
+    elif len(sys.argv) > 1 and sys.argv[1] == "pointq5":
+        point_quintic_cases()
     elif len(sys.argv) > 1 and sys.argv[1] == "pair":
         pair_cases()
     else:
