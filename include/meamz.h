@@ -130,6 +130,21 @@ const char* mz_last_error(void);
 int mz_objective(mz_ctx* ctx, const double* knots, double* F, double* errvec, double* cell_energy,
                  double* cell_stress, double* forces);
 
+/* mz_objective without any device or host allocation at call time: the
+ * per-evaluation buffers (knot vector, spline tables, pair force vectors,
+ * per-atom and per-chunk partials, residual outputs) live in a static 64 MiB
+ * device array held by every GPU the library runs on until the process ends.
+ * Same arguments, kernels and results as mz_objective (no captured graph).
+ * A database whose buffers exceed the array (about 24 bytes per pair plus
+ * 56 bytes per atom, plus the requested outputs) is rejected with
+ * MZ_ERR_CAPACITY.  Every call on one GPU uses the same array: calls that
+ * overlap on one GPU race (serialize them, one context stream at a time).
+ * No counterpart in the reference: for callers that cannot allocate device
+ * memory at evaluation time (code under stream capture, tight serving loops,
+ * harnesses that forbid hipMalloc). */
+int mz_objective_noalloc(mz_ctx* ctx, const double* knots, double* F, double* errvec, double* cell_energy,
+                         double* cell_stress, double* forces);
+
 /* Per-atom densities before clamping (MZ_ERR_LAYOUT for the pair styles,
  * which have none).
  * Replaces: compute_densities src/pots/pot_meam_spline/pot_meam_spline.cpp:79-96,289-349. */

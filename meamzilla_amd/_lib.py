@@ -79,6 +79,7 @@ def lib() -> C.CDLL:
     L.mz_destroy.argtypes = [vp]
     L.mz_info.argtypes = [vp, C.POINTER(Info)]
     L.mz_objective.argtypes = [vp, dp, dp, dp, dp, dp, dp]
+    L.mz_objective_noalloc.argtypes = [vp, dp, dp, dp, dp, dp, dp]
     L.mz_densities.argtypes = [vp, dp, dp]
     L.mz_objective_grad.argtypes = [vp, dp, dp, dp]
     L.mz_objective_batch.argtypes = [vp, C.c_int, dp, dp, dp]
@@ -185,6 +186,18 @@ class Objective:
         cs = np.zeros((self.ncells, 6)) if outputs else None
         fr = np.zeros((self.natoms, 3)) if outputs else None
         _check(lib().mz_objective(self._ctx, _dp(kv), _dp(F), _dp(ev), _dp(ce), _dp(cs), _dp(fr)))
+        return dict(F=float(F[0]), errvec=ev, cell_energy=ce, cell_stress=cs, forces=fr)
+
+    def compute_noalloc(self, knots: Optional[np.ndarray] = None, errvec: bool = True, outputs: bool = True) -> dict:
+        """compute() through mz_objective_noalloc: no allocation at call time,
+        buffers in the library's static device array."""
+        kv = self._kv(knots)
+        F = np.zeros(1)
+        ev = np.zeros(self.info.nerr) if errvec else None
+        ce = np.zeros(self.ncells) if outputs else None
+        cs = np.zeros((self.ncells, 6)) if outputs else None
+        fr = np.zeros((self.natoms, 3)) if outputs else None
+        _check(lib().mz_objective_noalloc(self._ctx, _dp(kv), _dp(F), _dp(ev), _dp(ce), _dp(cs), _dp(fr)))
         return dict(F=float(F[0]), errvec=ev, cell_energy=ce, cell_stress=cs, forces=fr)
 
     def objective(self, knots: Optional[np.ndarray] = None) -> float:

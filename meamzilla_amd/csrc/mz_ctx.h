@@ -57,6 +57,22 @@ struct Grid {
   double xcut() const { return x[n - 1] * (1. + 1.e-8); }
 };
 
+// Per-evaluation device buffers of the forward launch sequence (refresh,
+// atoms, residual, final), each sized for the candidates of one launch.  The
+// context owns one set (allocated on demand); the allocation-free entry point
+// carves another out of a static device array.
+struct EvalBufs {
+  double* knots;      // [ncand][nknot]
+  char* tables;       // [ncand][table_bytes]
+  double* forces;     // [ncand][natoms*3]
+  double* tvec;       // [ncand][npairs*3]
+  double* fown;       // [ncand][natoms*3]
+  double* chunk_acc;  // [ncand][nchunks][8]
+  double* partial;    // [ncand][nblocks]
+  double* F;          // [ncand]
+  int nblocks;        // residual workgroups per candidate
+};
+
 // Device state of the gradient path, sized for `cap` candidates.
 struct AdjointExtra {
   bool ready = false;

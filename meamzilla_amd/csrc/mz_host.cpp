@@ -439,7 +439,22 @@ mz::RefreshMeta refresh_meta(const mz_ctx* c) {
   return m;
 }
 
-mz::AtomArgs atom_args(const mz_ctx* c) {
+// The context's own per-evaluation buffers.
+mz::EvalBufs ctx_bufs(const mz_ctx* c) {
+  mz::EvalBufs b;
+  b.knots = c->d_knots.p;
+  b.tables = c->d_tables.p;
+  b.forces = c->d_forces.p;
+  b.tvec = c->d_tvec.p;
+  b.fown = c->d_fown.p;
+  b.chunk_acc = c->d_chunk_acc.p;
+  b.partial = c->d_partial.p;
+  b.F = c->d_F.p;
+  b.nblocks = c->resid_blocks;
+  return b;
+}
+
+mz::AtomArgs atom_args(const mz_ctx* c, const mz::EvalBufs& b) {
   mz::AtomArgs a{};
   a.pot = c->pot;
   a.chunks = c->d_chunks.p;
@@ -449,19 +464,19 @@ mz::AtomArgs atom_args(const mz_ctx* c) {
   a.box = c->d_box.p;
   a.types = c->d_types.p;
   a.weight = c->d_weight.p;
-  a.tables = c->d_tables.p;
+  a.tables = b.tables;
   a.table_bytes = c->table_bytes;
   a.natoms = c->natoms;
   a.ncells = c->ncells;
   a.npairs = c->npairs;
   a.nchunks = (int)c->chunks.size();
-  a.tvec = c->d_tvec.p;
-  a.fown = c->d_fown.p;
-  a.chunk_acc = c->d_chunk_acc.p;
+  a.tvec = b.tvec;
+  a.fown = b.fown;
+  a.chunk_acc = b.chunk_acc;
   return a;
 }
 
-mz::ResidArgs resid_args(const mz_ctx* c) {
+mz::ResidArgs resid_args(const mz_ctx* c, const mz::EvalBufs& b) {
   mz::ResidArgs r{};
   r.natoms = c->natoms;
   r.ncells = c->ncells;
@@ -481,22 +496,22 @@ mz::ResidArgs resid_args(const mz_ctx* c) {
   r.npairs = c->npairs;
   r.in_off = c->d_in_off.p;
   r.in_list = c->d_in_list.p;
-  r.tvec = c->d_tvec.p;
-  r.fown = c->d_fown.p;
-  r.forces = c->d_forces.p;
-  r.chunk_acc = c->d_chunk_acc.p;
+  r.tvec = b.tvec;
+  r.fown = b.fown;
+  r.forces = b.forces;
+  r.chunk_acc = b.chunk_acc;
   r.cell_chunk_off = c->d_cell_chunk_off.p;
   r.cell_chunks = c->d_cell_chunks.p;
   r.nchunks = (int)c->chunks.size();
-  r.partial = c->d_partial.p;
+  r.partial = b.partial;
   r.nodens = c->pot.nodens;
   return r;
 }
 
-mz::FinalArgs final_args(const mz_ctx* c) {
+mz::FinalArgs final_args(const mz_ctx* c, const mz::EvalBufs& b) {
   mz::FinalArgs f{};
-  f.npartial = c->resid_blocks;
-  f.partial = c->d_partial.p;
+  f.npartial = b.nblocks;
+  f.partial = b.partial;
   f.meam = c->meam;
   f.quintic = c->pot.quintic;
   f.ncells = c->ncells;
@@ -514,25 +529,28 @@ mz::FinalArgs final_args(const mz_ctx* c) {
   for (int g = 0; g < 3; ++g) f.pgrp0[g] = c->pot.fn_off[g == 0 ? 3 : 3 + 2 * g];
   f.koff = c->d_koff.p;
   f.nknots = c->d_nknots.p;
-  f.knots = c->d_knots.p;
-  f.tables = c->d_tables.p;
+  f.knots = b.knots;
+  f.tables = b.tables;
   f.table_bytes = c->table_bytes;
-  f.F = c->d_F.p;
+  f.F = b.F;
   return f;
 }
 
 }  // namespace
 
+int mz_noalloc_setup(int dev);
+
 // Launch sequence of one batch of ncand evaluations whose knot vectors are
-// already in d_knots.  Shared by the plain, gradient and batched entry points.
-int mz_run_forward(mz_ctx* c, int ncand, double* d_errvec, double* d_ce, double* d_cs, double* d_rho, double* d_dF,
-                   double* d_G, double* d_lam, double* d_cadj, double* d_T1) {
+// already in b.knots, on the per-evaluation buffers b.  Every buffer of b is
+// written before it is read: nothing needs clearing between calls.
+int mz_run_forward_on(mz_ctx* c, const mz::EvalBufs& b, int ncand, double* d_errvec, double* d_ce, double* d_cs,
+                      double* d_rho, double* d_dF, double* d_G, double* d_lam, double* d_cadj, double* d_T1) {
   hipStream_t st = c->stream;
   if (c->timing) MZ_HIP(hipEventRecord(c->ev[0], st));
-  mz::launch_refresh(st, c->d_knots.p, c->nknot, refresh_meta(c), c->d_tables.p, c->table_bytes, ncand);
+  mz::launch_refresh(st, b.knots, c->nknot, refresh_meta(c), b.tables, c->table_bytes, ncand);
   if (c->timing) MZ_HIP(hipEventRecord(c->ev[1], st));
   if (c->timing) MZ_HIP(hipEventRecord(c->ev[2], st));
-  mz::AtomArgs aa = atom_args(c);
+  mz::AtomArgs aa = atom_args(c, b);
   aa.rho_out = d_rho;
   aa.dF_out = d_dF;
   aa.G_out = d_G;
@@ -548,19 +566,26 @@ int mz_run_forward(mz_ctx* c, int ncand, double* d_errvec, double* d_ce, double*
     return fail(MZ_ERR_ARG, "unsupported tile (block, pairs per lane)");
   MZ_HIP(hipGetLastError());
   if (c->timing) MZ_HIP(hipEventRecord(c->ev[7], st));
-  mz::ResidArgs ra = resid_args(c);
+  mz::ResidArgs ra = resid_args(c, b);
   ra.errvec = d_errvec;
   ra.cell_energy = d_ce;
   ra.cell_stress = d_cs;
   ra.lam = d_lam;
   ra.cell_adj = d_cadj;
-  mz::launch_resid(st, ra, c->resid_blocks, ncand);
-  mz::FinalArgs fa = final_args(c);
+  mz::launch_resid(st, ra, b.nblocks, ncand);
+  mz::FinalArgs fa = final_args(c, b);
   fa.errvec = d_errvec;
   mz::launch_final(st, fa, ncand);
   MZ_HIP(hipGetLastError());
   if (c->timing) MZ_HIP(hipEventRecord(c->ev[4], st));
   return MZ_OK;
+}
+
+// The same sequence on the context's own buffers.  Shared by the plain,
+// gradient and batched entry points.
+int mz_run_forward(mz_ctx* c, int ncand, double* d_errvec, double* d_ce, double* d_cs, double* d_rho, double* d_dF,
+                   double* d_G, double* d_lam, double* d_cadj, double* d_T1) {
+  return mz_run_forward_on(c, ctx_bufs(c), ncand, d_errvec, d_ce, d_cs, d_rho, d_dF, d_G, d_lam, d_cadj, d_T1);
 }
 
 int mz_stage_knots(mz_ctx* c, int ncand, const double* knots) {
@@ -625,7 +650,7 @@ int mz_forward_graph(mz_ctx* c, int ncand, const double* knots, double* F) {
 }
 
 // Accessors used by the adjoint and batch translation units.
-mz::AtomArgs mz_atom_args(const mz_ctx* c) { return atom_args(c); }
+mz::AtomArgs mz_atom_args(const mz_ctx* c) { return atom_args(c, ctx_bufs(c)); }
 
 extern "C" {
 
@@ -689,6 +714,13 @@ static int create_impl(const mz_layout* layout, const mz_db* db, bool device_lis
   if (hipGetDevice(&c->device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess) {
     delete c;
     return fail(MZ_ERR_HIP, "no HIP device available");
+  }
+  // device address of the allocation-free entry point's scratch array, once per device
+  rc = mz_noalloc_setup(c->device);
+  if (rc) {
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return rc;
   }
   const auto t_list = std::chrono::steady_clock::now();
   // the triplet counts read the list on the host unless every triplet of an
