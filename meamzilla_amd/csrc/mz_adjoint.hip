@@ -33,7 +33,6 @@ namespace mz {
 
 constexpr int kAdjBlock = 256;   // threads (= pairs) per adjoint workgroup
 constexpr int kAdjAtoms = 64;    // atoms per adjoint chunk
-constexpr int kSlots = 64;       // global partial tables per candidate
 
 struct AdjArgs {
   PotDesc pot;
@@ -54,14 +53,14 @@ struct AdjArgs {
   const double* T1;        // jmeam: [cand][npairs][2] T1_p = sum_k q p_k per p p q term
   const double* lam;       // [cand][natoms*3]
   const double* cell_adj;  // [cand][ncells*8]
-  double* slots;           // [cand][kSlots][ntab]
+  double* slots;           // [cand][ntab][gridDim.x]: one partial table per workgroup, slot-major
   int ntab;                // ncoef*6 + nspl*6
   int gco0, ngco;          // first g interval record, number of g interval records
   int grep_log2;           // log2 of the lane replicas of the g-coefficient adjoints
   int nrad;                // interval records below the g records (phi, rho, U, f)
   int rrep_log2;           // log2 of the lane replicas of their adjoints, -1: none
   int rrep_stride;         // doubles per replica (nrad NC padded to 1 mod 32: replicas on distinct banks)
-  int nchunks, cpw;        // chunks, and consecutive chunks per workgroup
+  int nchunks;             // chunks; workgroup x takes x, x + gridDim.x, ...
 };
 
 struct ChainArgs {
@@ -79,7 +78,7 @@ struct ChainArgs {
   const double* xn;
   const double* J;         // per spline [2n][n+4]: d(ypp, yp4)/d(knot record)
   const int* joff;
-  const double* slots;
+  const double* tab;       // [cand][ntab] coefficient and head adjoints, summed over workgroups
   const double* knots;
   const char* tables;
   int table_bytes;
@@ -266,11 +265,13 @@ __global__ void __launch_bounds__(kAdjBlock) k_adjoint(const AdjArgs A) {
   const int nacc = A.ntab + (MEAM ? A.ngco * NC * R : 0);
   for (int q = tid; q < nacc; q += kAdjBlock) atab[q] = 0.0;
 
-  // A workgroup takes cpw consecutive chunks of one candidate (similar work in
-  // the decreasing-work order): table staging, zeroing, the g-replica
-  // reduction and the slot flush are paid once per workgroup.
-  const int cbeg = blockIdx.x * A.cpw, cend = min(cbeg + A.cpw, A.nchunks);
-  for (int ci = cbeg; ci < cend; ++ci) {
+  // The grid holds as many workgroups as the GPU keeps resident; workgroup x
+  // of a candidate takes chunks x, x + gridDim.x, ... of the decreasing-work
+  // order, so every workgroup gets about the same work.  Table staging,
+  // zeroing, the g-replica reduction and the partial-table store are paid
+  // once per workgroup, and each candidate's partial tables are summed by
+  // k_chain in a fixed order (no global atomics).
+  for (int ci = blockIdx.x; ci < A.nchunks; ci += gridDim.x) {
   const Chunk ch = A.chunks[ci];
   const int p = tid;
   const bool has = p < ch.np;
@@ -665,25 +666,35 @@ __global__ void __launch_bounds__(kAdjBlock) k_adjoint(const AdjArgs A) {
     }
   }
   __syncthreads();
-  double* slot = A.slots + ((size_t)cand * kSlots + (blockIdx.x % kSlots)) * A.ntab;
-  for (int q = tid; q < A.ntab; q += kAdjBlock) {
-    const double v = atab[q];
-    if (v != 0.0) unsafeAtomicAdd(slot + q, v);
-  }
+  double* slot = A.slots + (size_t)cand * A.ntab * gridDim.x + blockIdx.x;
+  for (int q = tid; q < A.ntab; q += kAdjBlock) slot[(size_t)q * gridDim.x] = atab[q];
 }
 
-// One 64-lane workgroup per (spline, candidate): slot reduction, penalties,
+// Sum of every candidate's per-workgroup partial tables: one 256-lane
+// workgroup per table entry, lane l adds slots l, l + 256, ..., a fixed
+// shuffle tree adds each wave's lanes and lane 0 the four waves in order, so
+// the order never changes between runs.
+__global__ void __launch_bounds__(256) k_slotsum(const double* slots, int ntab, int nslot, double* out) {
+  __shared__ double red[4];
+  const int q = blockIdx.x, cand = blockIdx.y, tid = threadIdx.x;
+  const double* s = slots + ((size_t)cand * ntab + q) * nslot;
+  double v = 0.0;
+  for (int sl = tid; sl < nslot; sl += 256) v += s[sl];
+  const double t = block_sum<256>(v, red);
+  if (tid == 0) out[(size_t)cand * ntab + q] = t;
+}
+
+// One 64-lane workgroup per (spline, candidate): summed adjoint table, penalties,
 // interval coefficients -> (y, y'', y'''') -> knot record.
 __global__ void __launch_bounds__(64) k_chain(const ChainArgs A) {
   const int s = blockIdx.x, cand = blockIdx.y, lane = threadIdx.x;
   const int n = A.nknots[s], ni = n - 1, off = A.coef_off[s];
   __shared__ double cb[(kMaxKnots - 1) * 6], hb[6], yb[kMaxKnots], db[kMaxKnots], qb[kMaxKnots],
       rb[kMaxKnots + 4];
-  const double* slots = A.slots + (size_t)cand * kSlots * A.ntab;
+  const double* tab = A.tab + (size_t)cand * A.ntab;
   for (int q = lane; q < ni * 6 + 6; q += 64) {
     const int idx = q < ni * 6 ? off * 6 + q : A.ncoef * 6 + s * 6 + (q - ni * 6);
-    double sacc = 0.0;
-    for (int sl = 0; sl < kSlots; ++sl) sacc += slots[(size_t)sl * A.ntab + idx];
+    const double sacc = tab[idx];
     if (q < ni * 6) cb[q] = sacc; else hb[q - ni * 6] = sacc;
   }
   __syncthreads();
@@ -887,6 +898,8 @@ void host_refresh(int basis, int n, double left, double right, int natL, int nat
     if (e_ != hipSuccess) return mz_fail(MZ_ERR_HIP, std::string(#call ": ") + hipGetErrorString(e_)); \
   } while (0)
 
+const void* adjoint_fn(const mz_ctx* c, int grep_log2);
+
 int setup_adjoint(mz_ctx* c) {
   mz::AdjointExtra& X = c->adjx;
   if (X.ready) return MZ_OK;
@@ -979,6 +992,14 @@ int setup_adjoint(mz_ctx* c) {
   for (int lg = 4; lg >= 0 && X.rrep_log2 < 0; --lg)
     if (X.rrep_stride * (1 << lg) <= mz::kPartnerFields * mz::kAdjBlock) X.rrep_log2 = lg;
   X.dyn_bytes = fixed + (c->meam ? X.ngco * nc_g * (1 << X.grep_log2) * 8 : 0);
+  // resident workgroups of this instantiation on the whole GPU
+  X.fn = adjoint_fn(c, X.grep_log2);
+  int wpc = 0, cus = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&wpc, X.fn, mz::kAdjBlock, X.dyn_bytes) != hipSuccess || wpc < 1)
+    return mz_fail(MZ_ERR_HIP, "occupancy query of the adjoint kernel failed");
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess || cus < 1)
+    return mz_fail(MZ_ERR_HIP, "compute-unit count query failed");
+  X.resident = wpc * cus;
   X.ready = true;
   return MZ_OK;
 }
@@ -989,7 +1010,8 @@ int ensure_adjoint_buffers(mz_ctx* c, int ncand) {
   const size_t nc = (size_t)ncand;
   bool bad = X.d_rho.alloc(nc * c->natoms) || X.d_dF.alloc(nc * c->natoms) || X.d_G.alloc(nc * c->npairs * 4) ||
              X.d_lam.alloc(nc * c->natoms * 3) || X.d_cadj.alloc(nc * c->ncells * 8) ||
-             X.d_slots.alloc(nc * mz::kSlots * X.ntab) || X.d_grad.alloc(nc * c->nknot) ||
+             X.d_slots.alloc(std::max<size_t>(nc, 8 * (size_t)X.resident) * X.ntab) || X.d_tabsum.alloc(nc * X.ntab) ||
+             X.d_grad.alloc(nc * c->nknot) ||
              (c->pot.njm && X.d_T1.alloc(nc * c->npairs * 2));
   if (bad) return mz_fail(MZ_ERR_NOMEM, "device allocation of gradient buffers failed");
   if (X.h_grad) (void)hipHostFree(X.h_grad);
@@ -1003,23 +1025,47 @@ int ensure_adjoint_buffers(mz_ctx* c, int ncand) {
   return MZ_OK;
 }
 
+// The k_adjoint instantiation of a potential: style flags and the lane
+// replica count of the g-coefficient adjoints.
 template <bool M, bool P, bool Q, bool GP, int JM>
-void launch_adjoint_g(hipStream_t st, const mz::AdjArgs& a, int nchunks, int ncand, int dyn) {
-  const dim3 grid((nchunks + a.cpw - 1) / a.cpw, ncand), block(mz::kAdjBlock);
-  if (a.grep_log2 == 5) hipLaunchKernelGGL((mz::k_adjoint<M, P, Q, 32, GP, JM>), grid, block, dyn, st, a);
-  else if (a.grep_log2 == 3) hipLaunchKernelGGL((mz::k_adjoint<M, P, Q, 8, GP, JM>), grid, block, dyn, st, a);
-  else hipLaunchKernelGGL((mz::k_adjoint<M, P, Q, 1, GP, JM>), grid, block, dyn, st, a);
+const void* adjoint_fn_g(int grep_log2) {
+  if (grep_log2 == 5) return reinterpret_cast<const void*>(&mz::k_adjoint<M, P, Q, 32, GP, JM>);
+  if (grep_log2 == 3) return reinterpret_cast<const void*>(&mz::k_adjoint<M, P, Q, 8, GP, JM>);
+  return reinterpret_cast<const void*>(&mz::k_adjoint<M, P, Q, 1, GP, JM>);
 }
 
 template <bool M, bool P, bool Q>
-void launch_adjoint_t(hipStream_t st, const mz::AdjArgs& a, int nchunks, int ncand, int dyn) {
-  if (!M)
-    hipLaunchKernelGGL((mz::k_adjoint<M, P, Q, 1, false, 0>), dim3((nchunks + a.cpw - 1) / a.cpw, ncand),
-                       dim3(mz::kAdjBlock), dyn, st, a);
-  else if (a.pot.gpair) launch_adjoint_g<M, P, Q, true, 0>(st, a, nchunks, ncand, dyn);
-  else if (a.pot.njm == 1) launch_adjoint_g<M, P, Q, false, 1>(st, a, nchunks, ncand, dyn);
-  else if (a.pot.njm == 2) launch_adjoint_g<M, P, Q, false, 2>(st, a, nchunks, ncand, dyn);
-  else launch_adjoint_g<M, P, Q, false, 0>(st, a, nchunks, ncand, dyn);
+const void* adjoint_fn_t(const mz::PotDesc& pot, int grep_log2) {
+  if (!M) return reinterpret_cast<const void*>(&mz::k_adjoint<M, P, Q, 1, false, 0>);
+  if (pot.gpair) return adjoint_fn_g<M, P, Q, true, 0>(grep_log2);
+  if (pot.njm == 1) return adjoint_fn_g<M, P, Q, false, 1>(grep_log2);
+  if (pot.njm == 2) return adjoint_fn_g<M, P, Q, false, 2>(grep_log2);
+  return adjoint_fn_g<M, P, Q, false, 0>(grep_log2);
+}
+
+const void* adjoint_fn(const mz_ctx* c, int grep_log2) {
+  const bool q = c->pot.quintic != 0;
+  if (c->meam) {
+    if (c->point) return q ? adjoint_fn_t<true, true, true>(c->pot, grep_log2) : adjoint_fn_t<true, true, false>(c->pot, grep_log2);
+    return q ? adjoint_fn_t<true, false, true>(c->pot, grep_log2) : adjoint_fn_t<true, false, false>(c->pot, grep_log2);
+  }
+  if (c->point) return q ? adjoint_fn_t<false, true, true>(c->pot, grep_log2) : adjoint_fn_t<false, true, false>(c->pot, grep_log2);
+  return q ? adjoint_fn_t<false, false, true>(c->pot, grep_log2) : adjoint_fn_t<false, false, false>(c->pot, grep_log2);
+}
+
+// Adjoint workgroups per candidate.  The launch holds eight times the
+// workgroups the GPU keeps resident (dispatch then evens out the progress of
+// the statically assigned chunk sequences: measured against one and four
+// times, 4.86 / 5.27 / 4.96 ms on the one-element 256-cell workload at 64
+// candidates), at most twice that for a single candidate (the partial tables
+// grow with the grid) and no more than there are chunks.  A multiple of eight
+// puts workgroup x of every candidate, which walks the same chunks, on the
+// same XCD (workgroups go to the eight XCDs round-robin).
+int adjoint_nslot(const mz::AdjointExtra& X, int ncand) {
+  const int nc = std::max(ncand, 1);
+  int n = std::min(8 * X.resident / nc, 2 * X.resident);
+  if (n >= 8) n &= ~7;
+  return std::max(1, std::min(X.nchunks, n));
 }
 
 }  // namespace
@@ -1029,6 +1075,8 @@ int mz_run_forward(mz_ctx* c, int ncand, double* d_errvec, double* d_ce, double*
 mz::AtomArgs mz_atom_args(const mz_ctx* c);
 void mz_record_timing(mz_ctx* c);
 int mz_ensure_buffers(mz_ctx* c, int ncand);
+size_t mz_candidate_budget(const mz_ctx* c);
+size_t mz_forward_bytes(const mz_ctx* c);
 
 // Launch sequence of objective + gradient for ncand staged knot vectors.
 static int run_grad(mz_ctx* c, int ncand) {
@@ -1037,7 +1085,6 @@ static int run_grad(mz_ctx* c, int ncand) {
   int rc = mz_run_forward(c, ncand, nullptr, nullptr, nullptr, X.d_rho.p, X.d_dF.p, c->meam ? X.d_G.p : nullptr,
                           X.d_lam.p, X.d_cadj.p, c->pot.njm ? X.d_T1.p : nullptr);
   if (rc) return rc;
-  mz::launch_zero(st, X.d_slots.p, (size_t)ncand * mz::kSlots * X.ntab);
   if (c->timing) ADJ_HIP(hipEventRecord(c->ev[5], st));
   mz::AdjArgs a{};
   a.pot = c->pot;
@@ -1068,24 +1115,12 @@ static int run_grad(mz_ctx* c, int ncand) {
   a.rrep_log2 = X.rrep_log2;
   a.rrep_stride = X.rrep_stride;
   a.nchunks = X.nchunks;
-  // two chunks per workgroup while the grid keeps >= 8 rounds of 1024
-  // workgroups (measured: 1, 2, 4, 8 chunks -> 5.30, 4.98, 5.23, 5.19 ms on
-  // the one-element 256-cell workload at 64 candidates)
-  a.cpw = (long long)X.nchunks * ncand >= 2LL * 8 * 1024 ? 2 : 1;
-  const bool q = c->pot.quintic != 0;
-  if (c->meam) {
-    if (c->point) { if (q) launch_adjoint_t<true, true, true>(st, a, X.nchunks, ncand, X.dyn_bytes);
-                    else launch_adjoint_t<true, true, false>(st, a, X.nchunks, ncand, X.dyn_bytes); }
-    else { if (q) launch_adjoint_t<true, false, true>(st, a, X.nchunks, ncand, X.dyn_bytes);
-           else launch_adjoint_t<true, false, false>(st, a, X.nchunks, ncand, X.dyn_bytes); }
-  } else {
-    if (c->point) { if (q) launch_adjoint_t<false, true, true>(st, a, X.nchunks, ncand, X.dyn_bytes);
-                    else launch_adjoint_t<false, true, false>(st, a, X.nchunks, ncand, X.dyn_bytes); }
-    else { if (q) launch_adjoint_t<false, false, true>(st, a, X.nchunks, ncand, X.dyn_bytes);
-           else launch_adjoint_t<false, false, false>(st, a, X.nchunks, ncand, X.dyn_bytes); }
-  }
-  ADJ_HIP(hipGetLastError());
+  const int nslot = adjoint_nslot(X, ncand);
+  void* kargs[] = {&a};
+  ADJ_HIP(hipLaunchKernel(X.fn, dim3(nslot, ncand), dim3(mz::kAdjBlock), kargs, X.dyn_bytes, st));
   if (c->timing) ADJ_HIP(hipEventRecord(c->ev[6], st));
+  hipLaunchKernelGGL(mz::k_slotsum, dim3(X.ntab, ncand), dim3(256), 0, st, X.d_slots.p, X.ntab, nslot, X.d_tabsum.p);
+  ADJ_HIP(hipGetLastError());
   mz::ChainArgs ca{};
   ca.nspl = c->nspl;
   ca.ncoef = c->ncoef;
@@ -1108,7 +1143,7 @@ static int run_grad(mz_ctx* c, int ncand) {
   ca.xn = c->d_xn.p;
   ca.J = X.d_J.p;
   ca.joff = X.d_joff.p;
-  ca.slots = X.d_slots.p;
+  ca.tab = X.d_tabsum.p;
   ca.knots = c->d_knots.p;
   ca.tables = c->d_tables.p;
   ca.table_bytes = c->table_bytes;
@@ -1123,10 +1158,12 @@ int mz_grad_impl(mz_ctx* c, int ncand, const double* knots, double* F, double* G
   int rc = setup_adjoint(c);
   if (rc) return rc;
   mz::AdjointExtra& X = c->adjx;
-  // bound the memory of the stored leg vectors: sub-batches of candidates
-  const size_t per_cand = (size_t)c->npairs * (c->pot.njm ? 9 : 7) * sizeof(double) + (size_t)c->natoms * 120 + (size_t)c->ncells * 64 +
-                          c->chunks.size() * 64 + (size_t)mz::kSlots * X.ntab * 8;
-  int sub = (int)std::max<size_t>(1, std::min<size_t>((size_t)ncand, (size_t)(48ull << 30) / std::max<size_t>(per_cand, 1)));
+  // sub-batches of candidates whose forward and gradient buffers (the stored
+  // leg vectors G_p above all) fit the device memory that is free
+  const size_t per_cand = mz_forward_bytes(c) + (size_t)c->npairs * (c->pot.njm ? 6 : 4) * sizeof(double) +
+                          (size_t)c->natoms * 40 + (size_t)c->ncells * 64 + (size_t)c->nknot * 8 +
+                          (size_t)X.ntab * 8 * std::max(1, 8 * X.resident / std::max(ncand, 1));
+  int sub = (int)std::max<size_t>(1, std::min<size_t>((size_t)ncand, mz_candidate_budget(c) / per_cand));
   mz_timing acc{};  // kernel times summed over the sub-batches of a timed call
   for (int c0 = 0; c0 < ncand; c0 += sub) {
     const int nb = std::min(sub, ncand - c0);
@@ -1194,7 +1231,7 @@ void mz_adjoint_drop(mz_ctx* c) {
     if (g.ex) (void)hipGraphExecDestroy(g.ex);
   X.graphs.clear();
   X.d_rho.release(); X.d_dF.release(); X.d_G.release(); X.d_lam.release(); X.d_cadj.release(); X.d_T1.release();
-  X.d_slots.release(); X.d_grad.release(); X.d_J.release(); X.d_joff.release(); X.d_chunks.release();
+  X.d_slots.release(); X.d_tabsum.release(); X.d_grad.release(); X.d_J.release(); X.d_joff.release(); X.d_chunks.release();
   X.d_owner.release();
   if (X.h_grad) (void)hipHostFree(X.h_grad);
   X.h_grad = nullptr;

@@ -79,11 +79,15 @@ struct AdjointExtra {
   int cap = 0;
   int nchunks = 0, ntab = 0, gco0 = 0, ngco = 0, dyn_bytes = 0, grep_log2 = 0, nrad = 0, rrep_log2 = -1,
       rrep_stride = 0;
+  const void* fn = nullptr;  // the k_adjoint instantiation of the potential
+  int resident = 0;          // its workgroups resident on the whole GPU
   DevBuf<Chunk> d_chunks;                 // chunks of <= 256 pairs
   DevBuf<int> d_owner;                    // owning atom of every pair
   DevBuf<double> d_J;                     // refresh Jacobians
   DevBuf<int> d_joff;
-  DevBuf<double> d_rho, d_dF, d_G, d_lam, d_cadj, d_slots, d_grad;
+  DevBuf<double> d_rho, d_dF, d_G, d_lam, d_cadj, d_grad;
+  DevBuf<double> d_slots;                 // [cand][ntab][workgroup] partial coefficient adjoints
+  DevBuf<double> d_tabsum;                // [cand][ntab] their sums
   DevBuf<double> d_T1;  // jmeam: [cap][npairs][2] sum_k q p_k of every pair
   double* h_grad = nullptr;               // pinned [cap][nknot]
   struct Graph {
@@ -117,6 +121,7 @@ struct mz_ctx {
   long long npairs = 0, ntriplets = 0;
   std::vector<mz::Chunk> chunks;
   std::vector<int> in_off, in_list;           // CSR: pairs whose neighbour is each atom
+  std::vector<int> in_pos;                    // slot of every pair in that CSR (host-built lists)
   std::vector<int> cell_chunk_off, cell_chunks; // CSR: chunks of every cell
   mz_tile tile{64, 512, 256, 2};
   // device: static
@@ -126,6 +131,7 @@ struct mz_ctx {
   mz::DevBuf<int> d_atom_poff, d_types, d_cell_of, d_natoms_c, d_abase;
   mz::DevBuf<uint2> d_pairs;
   mz::DevBuf<int> d_in_off, d_in_list, d_cell_chunk_off, d_cell_chunks;
+  mz::DevBuf<int> d_in_pos;  // [npairs] slot of every pair in the incoming-pair index
   mz::DevBuf<double> d_pos, d_box, d_weight, d_eweight, d_sweight, d_vol, d_energy0, d_stress0, d_force0;
   mz::DevBuf<int> d_nknots, d_basis, d_koff, d_bc, d_coef_off;
   mz::DevBuf<double> d_x0, d_xn;

@@ -470,6 +470,7 @@ mz::AtomArgs atom_args(const mz_ctx* c, const mz::EvalBufs& b) {
   a.ncells = c->ncells;
   a.npairs = c->npairs;
   a.nchunks = (int)c->chunks.size();
+  a.in_pos = c->d_in_pos.p;
   a.tvec = b.tvec;
   a.fown = b.fown;
   a.chunk_acc = b.chunk_acc;
@@ -495,7 +496,6 @@ mz::ResidArgs resid_args(const mz_ctx* c, const mz::EvalBufs& b) {
   r.stress_weight = c->stress_weight;
   r.npairs = c->npairs;
   r.in_off = c->d_in_off.p;
-  r.in_list = c->d_in_list.p;
   r.tvec = b.tvec;
   r.fown = b.fown;
   r.forces = b.forces;
@@ -802,6 +802,8 @@ static int create_impl(const mz_layout* layout, const mz_db* db, bool device_lis
     c->in_list.assign(np, 0);
     std::vector<int> fill(c->in_off.begin(), c->in_off.end() - 1);
     for (long long q = 0; q < np; ++q) c->in_list[fill[c->pairs[q].x]++] = (int)q;
+    c->in_pos.assign(np, 0);
+    for (long long q = 0; q < np; ++q) c->in_pos[c->in_list[q]] = (int)q;
   }
 
   // device upload (the device builder already placed the pair list and offsets)
@@ -811,7 +813,8 @@ static int create_impl(const mz_layout* layout, const mz_db* db, bool device_lis
   bool bad = (!device_lists && (c->d_atom_poff.upload(c->atom_poff.data(), c->atom_poff.size()) ||
                                  c->d_pairs.upload(c->pairs.data(), c->pairs.size()) ||
                                  c->d_in_off.upload(c->in_off.data(), c->in_off.size()) ||
-                                 c->d_in_list.upload(c->in_list.data(), c->in_list.size()))) ||
+                                 c->d_in_list.upload(c->in_list.data(), c->in_list.size()) ||
+                                 c->d_in_pos.upload(c->in_pos.data(), c->in_pos.size()))) ||
              c->d_pos.upload(db->pos, 3 * (size_t)na) || c->d_box.upload(db->box, 9 * (size_t)c->ncells) ||
              c->d_types.upload(c->types.data(), na) || c->d_cell_of.upload(c->cell_of.data(), na) ||
              c->d_natoms_c.upload(c->natoms_c.data(), c->ncells) || c->d_abase.upload(c->abase.data(), c->ncells) ||
@@ -857,7 +860,7 @@ void mz_destroy(mz_ctx* c) {
                              &c->d_forces, &c->d_tvec, &c->d_fown, &c->d_chunk_acc, &c->d_partial, &c->d_F, &c->d_errvec, &c->d_cell_e,
                              &c->d_cell_s, &c->d_rho, &c->d_dF, &c->d_G};
   for (auto* b : dbufs) b->release();
-  DevBuf<int>* ibufs[] = {&c->d_in_off, &c->d_in_list, &c->d_cell_chunk_off, &c->d_cell_chunks, &c->d_atom_poff, &c->d_types, &c->d_cell_of, &c->d_natoms_c, &c->d_abase,
+  DevBuf<int>* ibufs[] = {&c->d_in_off, &c->d_in_list, &c->d_in_pos, &c->d_cell_chunk_off, &c->d_cell_chunks, &c->d_atom_poff, &c->d_types, &c->d_cell_of, &c->d_natoms_c, &c->d_abase,
                           &c->d_nknots, &c->d_basis, &c->d_koff, &c->d_bc, &c->d_coef_off};
   for (auto* b : ibufs) b->release();
   c->d_pairs.release();
@@ -983,17 +986,42 @@ int mz_last_timing(const mz_ctx* c, mz_timing* out) {
 
 int mz_ensure_buffers(mz_ctx* c, int ncand) { return ensure_eval_buffers(c, ncand); }
 
+// Device bytes that the per-candidate buffers of one launch may take: what is
+// free now plus what the context's per-candidate buffers already hold (a
+// larger batch reallocates them), less a reserve of 1/16 of the device (at
+// least 2 GiB), and at most three quarters of the device.
+size_t mz_candidate_budget(const mz_ctx* c) {
+  size_t freeb = 0, totalb = 0;
+  if (hipMemGetInfo(&freeb, &totalb) != hipSuccess || totalb == 0) return size_t(16) << 30;
+  const mz::AdjointExtra& X = c->adjx;
+  const size_t held = sizeof(double) * (c->d_knots.n + c->d_forces.n + c->d_tvec.n + c->d_fown.n + c->d_chunk_acc.n +
+                                        c->d_partial.n + c->d_F.n + c->d_errvec.n + X.d_rho.n + X.d_dF.n + X.d_G.n +
+                                        X.d_lam.n + X.d_cadj.n + X.d_slots.n + X.d_tabsum.n + X.d_grad.n + X.d_T1.n) +
+                      c->d_tables.n;
+  const size_t reserve = std::max<size_t>(size_t(2) << 30, totalb / 16);
+  const size_t avail = freeb + held > reserve ? freeb + held - reserve : 0;
+  return std::max<size_t>(size_t(1) << 30, std::min(avail, totalb / 4 * 3));
+}
+
+// Per-candidate device bytes of the forward launch sequence.
+size_t mz_forward_bytes(const mz_ctx* c) {
+  return (size_t)c->npairs * 24 + (size_t)c->natoms * 48 + c->chunks.size() * 64 + (size_t)c->nknot * 8 +
+         (size_t)c->table_bytes + (size_t)mz::resid_blocks(c->natoms, c->ncells) * 8 + 64;
+}
+
+size_t mz_candidate_budget(const mz_ctx* c);
+size_t mz_forward_bytes(const mz_ctx* c);
+
 // Batched scoring of ncand knot vectors in one launch sequence (forward only
 // when G is NULL; the gradient path lives in mz_adjoint.hip).
 
 int mz_batch_impl(mz_ctx* c, int ncand, const double* knots, double* F, double* G) {
   if (!c || !knots || !F || ncand < 1) return fail(MZ_ERR_ARG, "null argument or ncand < 1");
   if (G) return mz_grad_impl(c, ncand, knots, F, G);
-  // candidates go through in sub-batches that keep the per-candidate pair
-  // buffers within a fixed share of HBM
-  const size_t per = (size_t)c->npairs * 24 + (size_t)c->natoms * 72 + c->chunks.size() * 64 +
-                     (size_t)c->nknot * 8 + (size_t)c->table_bytes + 64;
-  const int sub = (int)std::max<size_t>(1, std::min<size_t>((size_t)ncand, (48ull << 30) / per));
+  // candidates go through in sub-batches whose per-candidate buffers fit the
+  // device memory that is free
+  const size_t per = mz_forward_bytes(c);
+  const int sub = (int)std::max<size_t>(1, std::min<size_t>((size_t)ncand, mz_candidate_budget(c) / per));
   int rc = ensure_eval_buffers(c, sub);
   if (rc) return rc;
   if (!c->timing) {
@@ -1017,12 +1045,11 @@ int mz_batch_impl(mz_ctx* c, int ncand, const double* knots, double* F, double* 
 
 // Residual vectors of ncand knot vectors: the forward launch sequence with the
 // residual output enabled for every candidate, in sub-batches that keep the
-// per-candidate buffers and residual vectors within a fixed share of HBM.
+// per-candidate buffers and residual vectors within the free device memory.
 int mz_residuals_batch_impl(mz_ctx* c, int ncand, const double* knots, double* F, double* errvec) {
   if (!c || !knots || !F || !errvec || ncand < 1) return fail(MZ_ERR_ARG, "null argument or ncand < 1");
-  const size_t per = (size_t)c->npairs * 24 + (size_t)c->natoms * 72 + c->chunks.size() * 64 +
-                     (size_t)c->nknot * 8 + (size_t)c->table_bytes + (size_t)c->nerr * 8 + 64;
-  const int sub = (int)std::max<size_t>(1, std::min<size_t>((size_t)ncand, (48ull << 30) / per));
+  const size_t per = mz_forward_bytes(c) + (size_t)c->nerr * 8;
+  const int sub = (int)std::max<size_t>(1, std::min<size_t>((size_t)ncand, mz_candidate_budget(c) / per));
   int rc = ensure_eval_buffers(c, sub);
   if (rc) return rc;
   if (c->d_errvec.alloc((size_t)sub * c->nerr)) return fail(MZ_ERR_NOMEM, "residual buffer allocation failed");

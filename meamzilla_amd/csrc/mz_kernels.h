@@ -35,7 +35,8 @@ struct AtomArgs {
   int ncells;
   long long npairs;
   int nchunks;
-  double* tvec;           // [ncand][npairs*3] total force vector of every pair
+  const int* in_pos;      // [npairs] slot of every pair in the incoming-pair index of its neighbour
+  double* tvec;           // [ncand][npairs][3] total force vector of every pair, at its incoming slot
   double* fown;           // [ncand][natoms*3] sum of T over each atom's own pairs
   double* chunk_acc;      // [ncand][nchunks][8]: E, S(6), constraint of each chunk
   double* rho_out;        // optional [ncand][natoms] density before clamping
@@ -60,8 +61,7 @@ struct ResidArgs {
   double energy_weight, stress_weight;
   long long npairs;
   const int* in_off;            // [natoms+1] CSR of the pairs whose neighbour is each atom
-  const int* in_list;
-  const double* tvec;           // [ncand][npairs*3] force vector of every pair, on its owner
+  const double* tvec;           // [ncand][npairs][3] force vector of every pair, in incoming-index order
   const double* fown;           // [ncand][natoms*3] sum of T over each atom's own pairs
   double* forces;               // [ncand][natoms*3] F_a = own sum - incoming sum
   const double* chunk_acc;      // [ncand][nchunks][8]

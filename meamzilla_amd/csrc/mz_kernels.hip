@@ -670,7 +670,9 @@ __global__ void __launch_bounds__(BLOCK) k_atoms(const AtomArgs args) {
       if (valid[c]) {
         if (args.G_out)
           reinterpret_cast<double4*>(args.G_out)[pg] = make_double4(Gx[s][c], Gy[s][c], Gz[s][c], S1v[s][c]);
-        double* tv = args.tvec + pg * 3;
+        // stored at the pair's slot in its neighbour's incoming run, which
+        // k_resid then reads as one contiguous stream per atom
+        double* tv = args.tvec + ((size_t)cand[c] * args.npairs + args.in_pos[ch.p0 + p]) * 3;
         tv[0] = tx;
         tv[1] = ty;
         tv[2] = tz;
@@ -740,7 +742,7 @@ __global__ void __launch_bounds__(BLOCK) k_resid(const ResidArgs a) {
     double sx = 0.0, sy = 0.0, sz = 0.0;
     const int b = a.in_off[i], e = a.in_off[i + 1];
     for (int q = b + l; q < e; q += kResidLanes) {
-      const double* t = tv + 3 * (size_t)a.in_list[q];
+      const double* t = tv + 3 * (size_t)q;
       sx += t[0];
       sy += t[1];
       sz += t[2];

@@ -145,6 +145,12 @@ __global__ void __launch_bounds__(256) k_in_fill(const uint2* pairs, long long n
   list[off[j] + atomicAdd(fill + j, 1)] = (int)q;
 }
 
+// Inverse of the incoming-pair index: the slot of every pair in it.
+__global__ void __launch_bounds__(256) k_in_pos(const int* list, long long np, int* pos) {
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < np) pos[list[q]] = (int)q;
+}
+
 __global__ void __launch_bounds__(256) k_in_sort(const int* off, int* list, int na) {
   const int a = blockIdx.x * blockDim.x + threadIdx.x;
   if (a >= na) return;
@@ -179,7 +185,8 @@ int build_incoming_device(mz_ctx* c, std::string& err) {
     DevBuf<int> cnt;
     ~Scratch() { cnt.release(); }
   } tmp;
-  if (tmp.cnt.alloc(na) || c->d_in_off.alloc((size_t)na + 1) || c->d_in_list.alloc((size_t)std::max(np, 1LL))) {
+  if (tmp.cnt.alloc(na) || c->d_in_off.alloc((size_t)na + 1) || c->d_in_list.alloc((size_t)std::max(np, 1LL)) ||
+      c->d_in_pos.alloc((size_t)std::max(np, 1LL))) {
     err = "incoming-pair index: device allocation failed";
     return MZ_ERR_NOMEM;
   }
@@ -198,6 +205,7 @@ int build_incoming_device(mz_ctx* c, std::string& err) {
                              c->d_in_list.p);
   hipLaunchKernelGGL(k_in_sort, dim3((unsigned)((na + 255) / 256)), dim3(256), 0, st, c->d_in_off.p, c->d_in_list.p,
                      na);
+  if (np) hipLaunchKernelGGL(k_in_pos, dim3(nbq), dim3(256), 0, st, c->d_in_list.p, np, c->d_in_pos.p);
   MZ_LCHECK(hipGetLastError());
   MZ_LCHECK(hipStreamSynchronize(st));
   return MZ_OK;
