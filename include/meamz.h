@@ -185,14 +185,18 @@ int mz_set_tile(mz_ctx* ctx, const mz_tile* tile);
 int mz_get_tile(const mz_ctx* ctx, mz_tile* tile);
 int mz_autotune(mz_ctx* ctx, int ncand, int reps, mz_tile* best);
 
-/* Device timing of the dominant kernels over the last evaluation made with
- * timing enabled (HIP events on the context stream). */
+/* Device timing of the dominant kernels over the last call made with timing
+ * enabled: HIP events recorded on the context stream around each kernel
+ * (with timing on, calls launch on the stream instead of replaying their
+ * captured graphs), summed over the launches of the call (a batch larger
+ * than the free device memory holds runs as equal sub-batches). */
 typedef struct {
   double atom_ms;       /* atom (pair + triplet + embedding + force) kernel */
   double resid_ms;      /* force gather, residual and reduction kernels */
   double refresh_ms;    /* spline refresh kernel */
   double adjoint_ms;    /* reverse sweep (gradient) kernel, if run */
   double total_ms;      /* whole evaluation on the stream */
+  int launches;         /* launch sequences the call ran (sub-batches) */
 } mz_timing;
 int mz_enable_timing(mz_ctx* ctx, int on);
 int mz_last_timing(const mz_ctx* ctx, mz_timing* out);

@@ -50,7 +50,7 @@ class Tile(C.Structure):
 
 class Timing(C.Structure):
     _fields_ = [("atom_ms", C.c_double), ("resid_ms", C.c_double), ("refresh_ms", C.c_double),
-                ("adjoint_ms", C.c_double), ("total_ms", C.c_double)]
+                ("adjoint_ms", C.c_double), ("total_ms", C.c_double), ("launches", C.c_int)]
 
 
 _lib = None

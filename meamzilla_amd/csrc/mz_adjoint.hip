@@ -1073,7 +1073,8 @@ int adjoint_nslot(const mz::AdjointExtra& X, int ncand) {
 int mz_run_forward(mz_ctx* c, int ncand, double* d_errvec, double* d_ce, double* d_cs, double* d_rho, double* d_dF,
                    double* d_G, double* d_lam, double* d_cadj, double* d_T1);
 mz::AtomArgs mz_atom_args(const mz_ctx* c);
-void mz_record_timing(mz_ctx* c);
+void mz_timing_add(mz_ctx* c, mz_timing* acc, bool adjoint);
+int mz_sub_batch(int ncand, size_t fit);
 int mz_ensure_buffers(mz_ctx* c, int ncand);
 size_t mz_candidate_budget(const mz_ctx* c);
 size_t mz_forward_bytes(const mz_ctx* c);
@@ -1150,6 +1151,7 @@ static int run_grad(mz_ctx* c, int ncand) {
   ca.grad = X.d_grad.p;
   hipLaunchKernelGGL(mz::k_chain, dim3(c->nspl, ncand), dim3(64), 0, st, ca);
   ADJ_HIP(hipGetLastError());
+  if (c->timing) ADJ_HIP(hipEventRecord(c->ev[3], st));
   return MZ_OK;
 }
 
@@ -1163,7 +1165,7 @@ int mz_grad_impl(mz_ctx* c, int ncand, const double* knots, double* F, double* G
   const size_t per_cand = mz_forward_bytes(c) + (size_t)c->npairs * (c->pot.njm ? 6 : 4) * sizeof(double) +
                           (size_t)c->natoms * 40 + (size_t)c->ncells * 64 + (size_t)c->nknot * 8 +
                           (size_t)X.ntab * 8 * std::max(1, 8 * X.resident / std::max(ncand, 1));
-  int sub = (int)std::max<size_t>(1, std::min<size_t>((size_t)ncand, mz_candidate_budget(c) / per_cand));
+  const int sub = mz_sub_batch(ncand, mz_candidate_budget(c) / per_cand);
   mz_timing acc{};  // kernel times summed over the sub-batches of a timed call
   for (int c0 = 0; c0 < ncand; c0 += sub) {
     const int nb = std::min(sub, ncand - c0);
@@ -1178,21 +1180,14 @@ int mz_grad_impl(mz_ctx* c, int ncand, const double* knots, double* F, double* G
     std::memcpy(c->h_stage, kv, n * sizeof(double));
     hipStream_t st = c->stream;
     if (c->timing) {
+      // stream launches with the kernel events (graph replays keep no event timestamps)
       ADJ_HIP(hipMemcpyAsync(c->d_knots.p, c->h_stage, n * sizeof(double), hipMemcpyHostToDevice, st));
       rc = run_grad(c, nb);
       if (rc) return rc;
       ADJ_HIP(hipMemcpyAsync(c->h_F, c->d_F.p, nb * sizeof(double), hipMemcpyDeviceToHost, st));
       ADJ_HIP(hipMemcpyAsync(X.h_grad, X.d_grad.p, n * sizeof(double), hipMemcpyDeviceToHost, st));
       ADJ_HIP(hipStreamSynchronize(st));
-      float ms = 0;
-      (void)hipEventElapsedTime(&ms, c->ev[5], c->ev[6]);
-      mz_record_timing(c);
-      acc.atom_ms += c->last.atom_ms;
-      acc.resid_ms += c->last.resid_ms;
-      acc.refresh_ms += c->last.refresh_ms;
-      acc.total_ms += c->last.total_ms;
-      acc.adjoint_ms += ms;
-      c->last = acc;
+      mz_timing_add(c, &acc, true);
     } else {
       hipGraphExec_t ex = nullptr;
       for (auto& g : X.graphs)
@@ -1222,6 +1217,7 @@ int mz_grad_impl(mz_ctx* c, int ncand, const double* knots, double* F, double* G
     std::memcpy(F + c0, c->h_F, nb * sizeof(double));
     std::memcpy(G + (size_t)c0 * c->nknot, X.h_grad, n * sizeof(double));
   }
+  if (c->timing) c->last = acc;
   return MZ_OK;
 }
 

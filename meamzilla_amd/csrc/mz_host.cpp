@@ -597,40 +597,70 @@ int mz_stage_knots(mz_ctx* c, int ncand, const double* knots) {
   return MZ_OK;
 }
 
+// Adds the kernel times of the launch sequence that has just drained (events
+// recorded with timing on) to acc; `adjoint`: the sequence ran the reverse
+// sweep too (events 5-6 around k_adjoint, 3 at the end of the chain).
+void mz_timing_add(mz_ctx* c, mz_timing* acc, bool adjoint) {
+  if (!c->timing) return;
+  float t01 = 0, t27 = 0, t74 = 0, tend = 0, t56 = 0;
+  (void)hipEventElapsedTime(&t01, c->ev[0], c->ev[1]);
+  (void)hipEventElapsedTime(&t27, c->ev[2], c->ev[7]);
+  (void)hipEventElapsedTime(&t74, c->ev[7], c->ev[4]);
+  (void)hipEventElapsedTime(&tend, c->ev[0], c->ev[adjoint ? 3 : 4]);
+  if (adjoint) (void)hipEventElapsedTime(&t56, c->ev[5], c->ev[6]);
+  acc->refresh_ms += t01;
+  acc->atom_ms += t27;
+  acc->resid_ms += t74;
+  acc->total_ms += tend;
+  acc->adjoint_ms += t56;
+  acc->launches += 1;
+}
+
+// c->last from the one launch sequence that has just drained.
 void mz_record_timing(mz_ctx* c) {
   if (!c->timing) return;
-  float t01 = 0, t23 = 0, t34 = 0, t04 = 0;
-  (void)hipEventElapsedTime(&t01, c->ev[0], c->ev[1]);
-  (void)hipEventElapsedTime(&t23, c->ev[2], c->ev[7]);
-  (void)hipEventElapsedTime(&t34, c->ev[7], c->ev[4]);
-  (void)hipEventElapsedTime(&t04, c->ev[0], c->ev[4]);
-  c->last.refresh_ms = t01;
-  c->last.atom_ms = t23;
-  c->last.resid_ms = t34;
-  c->last.total_ms = t04;
+  c->last = mz_timing{};
+  mz_timing_add(c, &c->last, false);
+}
+
+// Equal sub-batches of at most `fit` candidates covering ncand.
+int mz_sub_batch(int ncand, size_t fit) {
+  const int cap = (int)std::max<size_t>(1, std::min<size_t>((size_t)ncand, fit));
+  const int parts = (ncand + cap - 1) / cap;
+  return (ncand + parts - 1) / parts;
 }
 
 // Forward objective of ncand knot vectors through a captured graph:
 // H2D copy of the staged knots, refresh, clears, atom kernel, residual and
 // final reduction, D2H copy of F.  One graph per candidate count; graphs are
-// dropped whenever buffers or chunks change.
-int mz_forward_graph(mz_ctx* c, int ncand, const double* knots, double* F) {
+// dropped whenever buffers or chunks change.  With timing on, the same
+// sequence is launched on the stream with its events (event timestamps are
+// not kept by graph replays) and the kernel times are added to acc.
+int mz_forward_graph(mz_ctx* c, int ncand, const double* knots, double* F, mz_timing* acc) {
   const size_t n = (size_t)ncand * c->nknot;
   for (size_t q = 0; q < n; ++q)
     if (!std::isfinite(knots[q])) return fail(MZ_ERR_ARG, "knot vector holds a non-finite value");
   std::memcpy(c->h_stage, knots, n * sizeof(double));
+  if (c->timing) {
+    MZ_HIP(hipMemcpyAsync(c->d_knots.p, c->h_stage, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    const int rc = mz_run_forward(c, ncand, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                                  nullptr);
+    if (rc) return rc;
+    MZ_HIP(hipMemcpyAsync(c->h_F, c->d_F.p, ncand * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    MZ_HIP(hipStreamSynchronize(c->stream));
+    if (acc) mz_timing_add(c, acc, false);
+    std::memcpy(F, c->h_F, ncand * sizeof(double));
+    return MZ_OK;
+  }
   hipGraphExec_t ex = nullptr;
   for (auto& g : c->graphs)
     if (g.ncand == ncand) ex = g.ex;
   if (!ex) {
     hipGraph_t graph = nullptr;
     MZ_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    const bool timing = c->timing;
-    c->timing = false;
     hipError_t e1 = hipMemcpyAsync(c->d_knots.p, c->h_stage, n * sizeof(double), hipMemcpyHostToDevice, c->stream);
     int rc = mz_run_forward(c, ncand, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
     hipError_t e2 = hipMemcpyAsync(c->h_F, c->d_F.p, ncand * sizeof(double), hipMemcpyDeviceToHost, c->stream);
-    c->timing = timing;
     hipError_t e3 = hipStreamEndCapture(c->stream, &graph);
     if (rc) return rc;
     if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess)
@@ -896,10 +926,12 @@ int mz_objective(mz_ctx* c, const double* knots, double* F, double* errvec, doub
   if (!c || !knots) return fail(MZ_ERR_ARG, "null argument");
   int rc = ensure_eval_buffers(c, 1);
   if (rc) return rc;
-  if (!errvec && !cell_energy && !cell_stress && !forces && !c->timing) {
+  if (!errvec && !cell_energy && !cell_stress && !forces) {
     double Fh = 0.0;
-    rc = mz_forward_graph(c, 1, knots, &Fh);
+    mz_timing acc{};
+    rc = mz_forward_graph(c, 1, knots, &Fh, &acc);
     if (!rc && F) *F = Fh;
+    if (!rc && c->timing) c->last = acc;
     return rc;
   }
   if (errvec && c->d_errvec.alloc(c->nerr)) return fail(MZ_ERR_NOMEM, "errvec allocation failed");
@@ -1011,6 +1043,7 @@ size_t mz_forward_bytes(const mz_ctx* c) {
 
 size_t mz_candidate_budget(const mz_ctx* c);
 size_t mz_forward_bytes(const mz_ctx* c);
+int mz_forward_graph(mz_ctx* c, int ncand, const double* knots, double* F, mz_timing* acc);
 
 // Batched scoring of ncand knot vectors in one launch sequence (forward only
 // when G is NULL; the gradient path lives in mz_adjoint.hip).
@@ -1018,28 +1051,18 @@ size_t mz_forward_bytes(const mz_ctx* c);
 int mz_batch_impl(mz_ctx* c, int ncand, const double* knots, double* F, double* G) {
   if (!c || !knots || !F || ncand < 1) return fail(MZ_ERR_ARG, "null argument or ncand < 1");
   if (G) return mz_grad_impl(c, ncand, knots, F, G);
-  // candidates go through in sub-batches whose per-candidate buffers fit the
-  // device memory that is free
-  const size_t per = mz_forward_bytes(c);
-  const int sub = (int)std::max<size_t>(1, std::min<size_t>((size_t)ncand, mz_candidate_budget(c) / per));
+  // candidates go through in equal sub-batches whose per-candidate buffers
+  // fit the device memory that is free
+  const int sub = mz_sub_batch(ncand, mz_candidate_budget(c) / mz_forward_bytes(c));
   int rc = ensure_eval_buffers(c, sub);
   if (rc) return rc;
-  if (!c->timing) {
-    for (int c0 = 0; c0 < ncand; c0 += sub) {
-      const int nb = std::min(sub, ncand - c0);
-      rc = mz_forward_graph(c, nb, knots + (size_t)c0 * c->nknot, F + c0);
-      if (rc) return rc;
-    }
-    return MZ_OK;
+  mz_timing acc{};
+  for (int c0 = 0; c0 < ncand; c0 += sub) {
+    const int nb = std::min(sub, ncand - c0);
+    rc = mz_forward_graph(c, nb, knots + (size_t)c0 * c->nknot, F + c0, &acc);
+    if (rc) return rc;
   }
-  if (sub < ncand) return fail(MZ_ERR_CAPACITY, "timed batch larger than one sub-batch");
-  rc = mz_stage_knots(c, ncand, knots);
-  if (rc) return rc;
-  rc = mz_run_forward(c, ncand, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
-  if (rc) return rc;
-  MZ_HIP(hipMemcpyAsync(F, c->d_F.p, sizeof(double) * ncand, hipMemcpyDeviceToHost, c->stream));
-  MZ_HIP(hipStreamSynchronize(c->stream));
-  mz_record_timing(c);
+  if (c->timing) c->last = acc;
   return MZ_OK;
 }
 
@@ -1048,12 +1071,12 @@ int mz_batch_impl(mz_ctx* c, int ncand, const double* knots, double* F, double* 
 // per-candidate buffers and residual vectors within the free device memory.
 int mz_residuals_batch_impl(mz_ctx* c, int ncand, const double* knots, double* F, double* errvec) {
   if (!c || !knots || !F || !errvec || ncand < 1) return fail(MZ_ERR_ARG, "null argument or ncand < 1");
-  const size_t per = mz_forward_bytes(c) + (size_t)c->nerr * 8;
-  const int sub = (int)std::max<size_t>(1, std::min<size_t>((size_t)ncand, mz_candidate_budget(c) / per));
+  const int sub = mz_sub_batch(ncand, mz_candidate_budget(c) / (mz_forward_bytes(c) + (size_t)c->nerr * 8));
   int rc = ensure_eval_buffers(c, sub);
   if (rc) return rc;
   if (c->d_errvec.alloc((size_t)sub * c->nerr)) return fail(MZ_ERR_NOMEM, "residual buffer allocation failed");
   hipStream_t st = c->stream;
+  mz_timing acc{};
   for (int c0 = 0; c0 < ncand; c0 += sub) {
     const int nb = std::min(sub, ncand - c0);
     rc = mz_stage_knots(c, nb, knots + (size_t)c0 * c->nknot);
@@ -1064,8 +1087,9 @@ int mz_residuals_batch_impl(mz_ctx* c, int ncand, const double* knots, double* F
     MZ_HIP(hipMemcpyAsync(errvec + (size_t)c0 * c->nerr, c->d_errvec.p, sizeof(double) * nb * c->nerr,
                           hipMemcpyDeviceToHost, st));
     MZ_HIP(hipStreamSynchronize(st));
-    mz_record_timing(c);
+    mz_timing_add(c, &acc, false);
   }
+  if (c->timing) c->last = acc;
   return MZ_OK;
 }
 

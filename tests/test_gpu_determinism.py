@@ -1,9 +1,10 @@
 """Run-to-run reproducibility: the forward path has fixed summation orders
 (no atomics), so two evaluations of the same knot vector give bitwise-equal
 objective, residual vector, per-cell energies and stresses and forces, plain
-and batched.  The gradient's coefficient adjoints are summed per workgroup
-in LDS and then across workgroups in a fixed order; two gradients agree
-component by component to round-off (1e-13 of each component's size)."""
+and batched.  The gradient's coefficient adjoints are summed per workgroup in
+LDS, where waves of one workgroup add in an order that can change between
+runs, and then across workgroups in a fixed order; two gradients agree to
+1e-13 of the gradient's size and to 1e-12 of every component's own size."""
 import numpy as np
 import pytest
 
@@ -17,6 +18,13 @@ KEYS = ("errvec", "cell_energy", "cell_stress", "forces")
 
 def _say(*a):
     print(*a, flush=True)
+
+
+def _close(g1, g2):
+    d = np.abs(g1 - g2)
+    top = np.max(np.abs(g1))
+    assert np.max(d) <= 1e-13 * top, np.max(d) / top
+    assert np.all(d <= 1e-12 * np.abs(g1) + 1e-14 * top), np.max(d / (np.abs(g1) + 1e-14 * top))
 
 
 def _workload(cfg, ncells, total):
@@ -48,12 +56,11 @@ def test_gradient_reproducible(cfg, ncells, total):
     F1, g1 = obj.objective_grad()
     F2, g2 = obj.objective_grad()
     assert F1 == F2
-    scale = np.abs(g1) + 1e-12 * np.max(np.abs(g1))
-    assert np.all(np.abs(g1 - g2) <= 1e-13 * scale), np.max(np.abs(g1 - g2) / scale)
+    _close(g1, g2)
     rng = np.random.Generator(np.random.PCG64(6))
     kv = w.pot.knot_vector() * (1.0 + 0.02 * rng.standard_normal((8, obj.info.nknot)))
     Fa, Ga = obj.batch(kv, grad=True)
     Fb, Gb = obj.batch(kv, grad=True)
     assert np.array_equal(Fa, Fb)
-    sc = np.abs(Ga) + 1e-12 * np.max(np.abs(Ga), axis=1, keepdims=True)
-    assert np.all(np.abs(Ga - Gb) <= 1e-13 * sc), np.max(np.abs(Ga - Gb) / sc)
+    for a, b in zip(Ga, Gb):
+        _close(a, b)
