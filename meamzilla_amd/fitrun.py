@@ -6,9 +6,10 @@ writes the workload as files in the reference's formats (configuration
 database with targets from the truth knot set, start potential with one knot
 of every rho and f spline fixed to remove the objective's scale gauges, one
 parameter file per optimizer), runs ``meamz_amd -p <param>`` for L-BFGS (adjoint
-gradient) and for Powell (residual Jacobian by forward differences, as in
-src/opt_powell.cpp:96-315) on the same objective, and prints one JSON line
-with each optimizer's objective trajectory, calls and wall time.
+gradient) and for the least-squares optimizer of optstyle powell
+(Levenberg-Marquardt on batched forward-difference Jacobians) on the same
+objective, and prints one JSON line with each optimizer's objective
+trajectory, calls and wall time.
 """
 from __future__ import annotations
 
@@ -36,9 +37,10 @@ def fix_gauge(pot) -> None:
     """Fixes the largest-magnitude knot of every rho and f spline.
 
     The MEAM objective is invariant under rho -> a rho (with U rescaled in
-    its argument) and f -> b f (with g -> g / b^2), so the Gauss-Newton system
-    of Powell's method is singular along those directions and its step
-    blows up; anchoring one knot per rho and f spline removes them."""
+    its argument) and f -> b f (with g -> g / b^2); anchoring one knot per rho
+    and f spline removes those flat directions, which slow L-BFGS and leave a
+    Gauss-Newton system singular (the least-squares optimizer's damping
+    copes with them, but the fit runs compare optimizers on one objective)."""
     counts = formats.function_counts(pot.style, pot.ntypes)
     off = [sum(counts[:i]) for i in range(len(counts))]
     idx = list(range(off[1], off[1] + counts[1])) if len(counts) > 1 else []
@@ -62,12 +64,10 @@ def _parse(text: str) -> dict:
         out["calls"], out["seconds"] = int(m.group(1)), float(m.group(2))
     for line in text.splitlines():
         f = line.split()
-        # L-BFGS: iter F |g| step seconds
+        # L-BFGS: iter F |g| step seconds; least squares: iter F lambda calls seconds
         if len(f) == 5 and f[0].isdigit():
             out["trajectory"].append({"iter": int(f[0]), "F": float(f[1]), "seconds": float(f[4])})
-        # Powell outer step: inner steps, calls, F
-        elif len(f) == 3 and f[0].isdigit() and f[1].isdigit():
-            out["trajectory"].append({"inner": int(f[0]), "calls": int(f[1]), "F": float(f[2])})
+        # least squares: iter F lambda calls seconds (parsed by the branch above)
         # genetic algorithm: generation g  best F = x
         elif len(f) == 6 and f[0] == "generation":
             out["trajectory"].append({"generation": int(f[1]), "F": float(f[5])})
@@ -88,7 +88,7 @@ def main(argv=None) -> int:
                     help="relative size of the random knot changes of the initial population")
     ap.add_argument("--out", required=True)
     ap.add_argument("--free-gauge", action="store_true",
-                    help="keep every knot free (Powell then meets the singular gauge directions)")
+                    help="keep every knot free (the objective then has flat gauge directions)")
     args = ap.parse_args(argv)
     if not os.path.exists(DRIVER):
         raise SystemExit(f"{DRIVER} is not built")

@@ -412,332 +412,220 @@ double Potential::compute_dir(ErrorVec* ev, const std::vector<double>& dir, doub
   return tmp.compute(ev);
 }
 
-// ---------------------------------------------------------------- line minimisation
-
-namespace {
-constexpr double CGOLD = 0.381966012;      // src/line_min.h:45
-constexpr unsigned MAX_BRACKET = 100;      // src/line_min.h:46
-constexpr unsigned ITMAX = 100;            // src/line_min_brent.h:29
-constexpr double TOL = 1.0e-1;             // src/line_min_brent.h:30
-constexpr double ZEPS = 1.0e-9;            // src/line_min_brent.h:31
-}  // namespace
-
-void LineMinBrent::bracket() {
-  double xl = 0.0, xm, xr = 0.1;
-  double fl = eval_(xl), fm, fr = eval_(xr);
-  if (fr >= fl) {
-    xm = xl;
-    fm = fl;
-    xl = -(xr - xm) / CGOLD + xr;
-    fl = eval_(xl);
-  } else {
-    xm = xr;
-    fm = fr;
-    xr = (xm - xl) / CGOLD + xl;
-    fr = eval_(xr);
-  }
-  for (unsigned count = 0; count < MAX_BRACKET; ++count) {
-    if (fm < fl) {
-      if (fm < fr) {
-        xa_ = xl; xb_ = xm; xc_ = xr;
-        fa_ = fl; fb_ = fm; fc_ = fr;
-        return;
-      } else if (fm > fr) {
-        xl = xm; fl = fm;
-        xm = xr; fm = fr;
-        xr = (xm - xl) / CGOLD + xl;
-        fr = eval_(xr);
-      } else {
-        xr = (xr - xl) * CGOLD + xr;
-        fr = eval_(xr);
-      }
-    } else if (fm > fl) {
-      xr = xm; fr = fm;
-      xm = xl; fm = fl;
-      xl = -(xr - xm) / CGOLD + xr;
-      fl = eval_(xl);
-    } else {
-      xl = -(xr - xl) * CGOLD + xl;
-      fl = eval_(xl);
-    }
-  }
-  throw Error("problems with bracketing of minimum");
-}
-
-double LineMinBrent::find_minima() {
-  double a = std::min(xa_, xc_), b = std::max(xa_, xc_);
-  double v = xb_, w = v, x = v, d = 0.0, e = 0.0;
-  double fx = fb_, fv = fx, fw = fx;
-  for (unsigned it = 0; it < ITMAX; ++it) {
-    double u = 0.0;
-    const double xm = 0.5 * (a + b);
-    const double tol1 = TOL * std::abs(x) + ZEPS, tol2 = 2. * tol1;
-    if (std::abs(x - xm) <= (tol2 - 0.5 * (b - a))) {
-      xmin1_ = x; xmin2_ = w; fmin1_ = fx; fmin2_ = fw;
-      return fmin1_;
-    }
-    if (std::abs(e) > tol1) {
-      double r = (x - w) * (fx - fv);
-      double q = (x - v) * (fx - fw);
-      double p = (x - v) * q - (x - w) * r;
-      q = 2. * (q - r);
-      if (q > 0.) p = -p; else q = -q;
-      const double etemp = e;
-      e = d;
-      if (std::abs(p) >= std::abs(0.5 * q * etemp) || p <= q * (a - x) || p >= q * (b - x)) {
-        e = (x < xm) ? b - x : -(x - a);
-        d = CGOLD * e;
-      } else {
-        d = p / q;
-        u = x + d;
-        if ((u - a) < tol2 || (b - u) < tol2) d = (x < xm) ? tol1 : -tol1;
-      }
-    } else {
-      e = (x < xm) ? b - x : -(x - a);
-      d = CGOLD * e;
-    }
-    u = (std::abs(d) >= tol1) ? x + d : x + ((d > 0) ? tol1 : -tol1);
-    const double fu = eval_(u);
-    if (fu <= fx) {
-      if (u >= x) a = x; else b = x;
-      v = w; fv = fw; w = x; fw = fx; x = u; fx = fu;
-    } else {
-      if (u < x) a = u; else b = u;
-      if (fu <= fw || w == x) { v = w; fv = fw; w = u; fw = fu; }
-      else if (fu <= fv || v == x || v == w) { v = u; fv = fu; }
-    }
-  }
-  throw Error("too many iterations in Brent minimization");
-}
-
-// ---------------------------------------------------------------- Powell
+// ---------------------------------------------------------------- least squares
 
 namespace {
 
-constexpr double P_EPS = 1.E-4;       // src/opt_powell.h:43-47
-constexpr double P_PRECISION = 1.E-7;
-constexpr double P_NOTHING = 1.E-12;
-constexpr int P_INNERLOOPS = 801;
-constexpr double P_TOOBIG = 1.E+4;
+constexpr int kBatch = 64;  // knot vectors per batched call of the optimizers
 
-// Solves the symmetric system A q = p by LU with partial pivoting and one
-// step of iterative refinement; returns forward and backward error estimates
-// in the sense of dsysvx (componentwise backward error, relative correction).
-bool sym_solve(int n, const std::vector<double>& A, const std::vector<double>& p, std::vector<double>& q,
-               double& ferr, double& berr) {
-  std::vector<double> LU = A;
-  std::vector<int> piv(n);
-  for (int k = 0; k < n; ++k) {
-    int pk = k;
-    for (int i = k + 1; i < n; ++i)
-      if (std::abs(LU[i * n + k]) > std::abs(LU[pk * n + k])) pk = i;
-    piv[k] = pk;
-    if (LU[pk * n + k] == 0.0) return false;
-    if (pk != k)
-      for (int c = 0; c < n; ++c) std::swap(LU[k * n + c], LU[pk * n + c]);
-    for (int i = k + 1; i < n; ++i) {
-      LU[i * n + k] /= LU[k * n + k];
-      for (int c = k + 1; c < n; ++c) LU[i * n + c] -= LU[i * n + k] * LU[k * n + c];
+// Knot vectors of several potentials, row-major, for one batched call.
+std::vector<double> knot_rows(const std::vector<const Potential*>& ps) {
+  std::vector<double> K;
+  for (const Potential* p : ps) {
+    const std::vector<double> kv = p->knot_vector();
+    K.insert(K.end(), kv.begin(), kv.end());
+  }
+  return K;
+}
+
+// Objectives of several potentials through mz_objective_batch; each
+// potential keeps its own (get_error).
+std::vector<double> batch_objective(mz_ctx* ctx, const std::vector<Potential*>& ps) {
+  std::vector<double> F(ps.size());
+  if (ps.empty()) return F;
+  std::vector<double> K = knot_rows(std::vector<const Potential*>(ps.begin(), ps.end()));
+  check(mz_objective_batch(ctx, (int)ps.size(), K.data(), F.data(), nullptr));
+  Potential::add_ncalls(ps.size());
+  for (size_t i = 0; i < ps.size(); ++i) ps[i]->set_error(F[i]);
+  return F;
+}
+
+// In-place Cholesky factor (lower triangle, row-major) of a symmetric positive
+// definite matrix; false when a pivot is not positive.
+bool cholesky(int n, std::vector<double>& M) {
+  for (int j = 0; j < n; ++j) {
+    double d = M[(size_t)j * n + j];
+    for (int k = 0; k < j; ++k) d -= M[(size_t)j * n + k] * M[(size_t)j * n + k];
+    if (!(d > 0.0)) return false;
+    d = std::sqrt(d);
+    M[(size_t)j * n + j] = d;
+    for (int i = j + 1; i < n; ++i) {
+      double t = M[(size_t)i * n + j];
+      for (int k = 0; k < j; ++k) t -= M[(size_t)i * n + k] * M[(size_t)j * n + k];
+      M[(size_t)i * n + j] = t / d;
     }
   }
-  auto solve = [&](std::vector<double> b) {
-    for (int k = 0; k < n; ++k) {
-      std::swap(b[k], b[piv[k]]);
-      for (int i = k + 1; i < n; ++i) b[i] -= LU[i * n + k] * b[k];
-    }
-    for (int k = n - 1; k >= 0; --k) {
-      for (int c = k + 1; c < n; ++c) b[k] -= LU[k * n + c] * b[c];
-      b[k] /= LU[k * n + k];
-    }
-    return b;
-  };
-  q = solve(p);
-  std::vector<double> r(n);
-  berr = 0.0;
-  for (int i = 0; i < n; ++i) {
-    double s = p[i], den = std::abs(p[i]);
-    for (int j = 0; j < n; ++j) {
-      s -= A[i * n + j] * q[j];
-      den += std::abs(A[i * n + j] * q[j]);
-    }
-    r[i] = s;
-    if (den > 0) berr = std::max(berr, std::abs(s) / den);
-  }
-  std::vector<double> dq = solve(r);
-  double nq = 0, nd = 0;
-  for (int i = 0; i < n; ++i) {
-    q[i] += dq[i];
-    nq = std::max(nq, std::abs(q[i]));
-    nd = std::max(nd, std::abs(dq[i]));
-  }
-  ferr = nq > 0 ? nd / nq : 0.0;
   return true;
 }
 
+// Solves L L^T x = b for the factor of cholesky().
+std::vector<double> cholesky_solve(int n, const std::vector<double>& L, std::vector<double> b) {
+  for (int i = 0; i < n; ++i) {
+    for (int k = 0; k < i; ++k) b[i] -= L[(size_t)i * n + k] * b[k];
+    b[i] /= L[(size_t)i * n + i];
+  }
+  for (int i = n - 1; i >= 0; --i) {
+    for (int k = i + 1; k < n; ++k) b[i] -= L[(size_t)k * n + i] * b[k];
+    b[i] /= L[(size_t)i * n + i];
+  }
+  return b;
+}
+
+// Runs body(i) for i in [0, n) on up to 16 host threads.
+template <class Body>
+void parallel_for(int n, Body body) {
+  const int nth = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  if (nth == 1 || n < 8) {
+    for (int i = 0; i < n; ++i) body(i);
+    return;
+  }
+  std::atomic<int> next{0};
+  std::vector<std::thread> th;
+  for (int t = 0; t < nth; ++t)
+    th.emplace_back([&]() {
+      for (int i = next++; i < n; i = next++) body(i);
+    });
+  for (auto& x : th) x.join();
+}
+
 }  // namespace
 
-Potential* powell(const Potential& start, const OptOptions& opt) {
+// Levenberg-Marquardt on the residual vector (Levenberg, Quart. Appl. Math. 2
+// (1944) 164-168; Marquardt, J. SIAM 11 (1963) 431-441, with Marquardt's
+// diagonal scaling).  Every iteration forms the Jacobian of the residuals by
+// forward differences -- all its columns from batched residual calls
+// (mz_residuals_batch, the perturbed knot vectors as candidates) -- and the
+// normal equations A = J^T J, g = J^T e; it then solves
+// (A + lambda diag(A)) delta = -g for four damping values lambda * {1/10, 1, 10,
+// 100}, scores the four trial potentials in one batched call and keeps the
+// lowest.  The damping follows the ratio of the actual to the predicted
+// decrease (F - |e + J delta|^2); a step that decreases nothing raises it a
+// hundredfold.  The damping also keeps the system regular along the
+// objective's scale gauges (rho -> a rho with U rescaled, f -> b f with g / b^2).
+Potential* least_squares(const Potential& start, const OptOptions& opt) {
   std::ostream* out = opt.out;
   Potential* pot = start.clone();
-  ErrorVec ev;
-  double F = pot->compute(&ev);
-  const int ncoeff = pot->get_ncoeff();
-  const int esize = (int)ev.size();
-  if (opt.max_steps == 0 || ncoeff == 0) return pot;
-  if (F < P_NOTHING) {
-    if (out) *out << "Error already too small to optimize, aborting..." << std::endl;
-    return pot;
-  }
-  std::vector<double> gamma((size_t)esize * ncoeff), d((size_t)ncoeff * ncoeff), les((size_t)ncoeff * ncoeff),
-      p(ncoeff), q(ncoeff), delta(ncoeff);
-  auto G = [&](int j, int i) -> double& { return gamma[(size_t)i * esize + j]; };
-  auto D = [&](int i, int j) -> double& { return d[(size_t)i * ncoeff + j]; };
-  // Forward-difference Jacobian of the residual vector, one column per
-  // coefficient (src/opt_powell.cpp:348-389); the ncoeff perturbed residual
-  // vectors come from batched calls of at most kJacBatch knot vectors.
-  constexpr int kJacBatch = 64;
-  auto gamma_init = [&](const Potential* base) -> int {
-    for (int i = 0; i < ncoeff; ++i)
-      for (int j = 0; j < ncoeff; ++j) D(i, j) = (i == j) ? 1. : 0.;
-    const std::vector<double> kv0 = base->knot_vector();
-    const std::vector<int>& slots = base->coeff_slots();
-    const size_t nk = kv0.size();
-    mz_ctx* ctx = base->evaluator()->ctx();
+  const int n = pot->get_ncoeff();
+  ErrorVec e;
+  double F = pot->compute(&e);
+  if (opt.max_steps == 0 || n == 0) return pot;
+  const size_t m = e.size();
+  mz_ctx* ctx = pot->evaluator()->ctx();
+  const int nk = pot->evaluator()->info().nknot;
+  std::vector<double> J(m * (size_t)n), A((size_t)n * n), g(n);  // J column-major: J[i*m + r] = de_r / dx_i
+  const double kFactors[4] = {0.1, 1.0, 10.0, 100.0};
+  double lambda = -1.0;
+  int fails = 0;
+  const auto t0 = std::chrono::steady_clock::now();
+  if (out) *out << "Levenberg-Marquardt least squares, " << n << " coefficients, " << m << " residuals" << std::endl
+                << "  iter        F             lambda        calls   seconds" << std::endl;
+  for (int it = 0; it < opt.max_steps; ++it) {
+    if (F < 1e-12) {
+      if (out) *out << "objective below 1e-12: nothing left to fit" << std::endl;
+      break;
+    }
+    // Jacobian columns: x_i + h_i, h_i = 1e-7 max(|x_i|, 1)
+    const std::vector<double> kv = pot->knot_vector();
+    const std::vector<int>& slots = pot->coeff_slots();
     std::vector<double> kb, Fb, Eb;
-    for (int i0 = 0; i0 < ncoeff; i0 += kJacBatch) {
-      const int nb = std::min(kJacBatch, ncoeff - i0);
-      kb.assign((size_t)nb * nk, 0.0);
+    for (int i0 = 0; i0 < n; i0 += kBatch) {
+      const int nb = std::min(kBatch, n - i0);
+      kb.resize((size_t)nb * nk);
+      std::vector<double> h(nb);
       for (int b = 0; b < nb; ++b) {
-        std::copy(kv0.begin(), kv0.end(), kb.begin() + (size_t)b * nk);
-        kb[(size_t)b * nk + slots[i0 + b]] += P_EPS;
+        std::copy(kv.begin(), kv.end(), kb.begin() + (size_t)b * nk);
+        const int sl = slots[i0 + b];
+        h[b] = 1e-7 * std::max(std::abs(kv[sl]), 1.0);
+        kb[(size_t)b * nk + sl] += h[b];
       }
       Fb.resize(nb);
-      Eb.resize((size_t)nb * esize);
+      Eb.resize((size_t)nb * m);
       if (mz_residuals_batch(ctx, nb, kb.data(), Fb.data(), Eb.data()) != MZ_OK)
         throw Error(std::string("mz_residuals_batch: ") + mz_last_error());
       Potential::add_ncalls(nb);
       for (int b = 0; b < nb; ++b) {
-        const int i = i0 + b;
-        const double* tv = Eb.data() + (size_t)b * esize;
-        double sum = 0.0;
-        for (int j = 0; j < esize; ++j) {
-          const double t = (tv[j] - ev[j]) / P_EPS;
-          G(j, i) = t;
-          sum += t * t;
-        }
-        const double mag = std::sqrt(sum);
-        if (mag < P_NOTHING) return i + 1;
-        for (int j = 0; j < esize; ++j) G(j, i) /= mag;
-        D(i, i) /= mag;
+        double* col = J.data() + (size_t)(i0 + b) * m;
+        const double* eb = Eb.data() + (size_t)b * m;
+        for (size_t r = 0; r < m; ++r) col[r] = (eb[r] - e[r]) / h[b];
       }
     }
-    return 0;
-  };
-  // Normal equations of the Gauss-Newton step: p = -Gamma^T e and
-  // les = Gamma^T Gamma (all of it, or one row and column).  The full product
-  // is O(ncoeff^2 esize) and is split over host threads by column.
-  auto lineqsys = [&](int only_col) {
-    auto column = [&](int i) {
-      p[i] = 0.;
-      for (int j = 0; j < esize; ++j) p[i] -= G(j, i) * ev[j];
-      if (only_col >= 0 && i != only_col) return;
-      for (int j = (only_col >= 0 ? 0 : i); j < ncoeff; ++j) {
-        double s = 0.0;
-        for (int k = 0; k < esize; ++k) s += G(k, i) * G(k, j);
-        les[(size_t)i * ncoeff + j] = s;
-        les[(size_t)j * ncoeff + i] = s;
+    // normal equations, one column of A per task
+    parallel_for(n, [&](int i) {
+      const double* ci = J.data() + (size_t)i * m;
+      double s = 0.0;
+      for (size_t r = 0; r < m; ++r) s += ci[r] * e[r];
+      g[i] = s;
+      for (int j = 0; j <= i; ++j) {
+        const double* cj = J.data() + (size_t)j * m;
+        double t = 0.0;
+        for (size_t r = 0; r < m; ++r) t += ci[r] * cj[r];
+        A[(size_t)i * n + j] = t;
+        A[(size_t)j * n + i] = t;
       }
-    };
-    const int nth = only_col >= 0 ? 1 : (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-    if (nth == 1) {
-      for (int i = 0; i < ncoeff; ++i) column(i);
-      return;
-    }
-    std::atomic<int> next{0};
-    std::vector<std::thread> th;
-    for (int t = 0; t < nth; ++t)
-      th.emplace_back([&]() {
-        for (int i = next++; i < ncoeff; i = next++) column(i);
-      });
-    for (auto& x : th) x.join();
-  };
-  if (out) *out << std::setw(10) << " " << std::setw(3) << "n" << std::setw(8) << "ncalls" << "   F" << std::endl;
-  double F_older = F;
-  int n = 0;
-  do {
-    const auto calls0 = Potential::get_ncalls();
-    if (int i = gamma_init(pot)) {
-      if (out) *out << "F does not depend on coefficient " << i - 1 << ", fit impossible!" << std::endl;
+    });
+    double dmax = 0.0;
+    for (int i = 0; i < n; ++i) dmax = std::max(dmax, A[(size_t)i * n + i]);
+    if (!(dmax > 0.0)) {
+      if (out) *out << "the residuals do not depend on the coefficients: nothing to fit" << std::endl;
       break;
     }
-    lineqsys(-1);
-    F_older = F;
-    int m = 0;
-    double dF = 0.0, delta_mag = 0.0;
-    do {
-      const double F_old = F;
-      double ferr = 0, berr = 0;
-      if (!sym_solve(ncoeff, les, p, q, ferr, berr)) {
-        if (out) *out << "Linear equation system singular after step " << m << std::endl;
+    if (lambda < 0) lambda = 1e-3;
+    // trial steps for the four damping values
+    std::vector<std::vector<double>> steps;
+    std::vector<double> lams, preds;
+    std::vector<Potential*> trials;
+    for (double fct : kFactors) {
+      const double lam = lambda * fct;
+      std::vector<double> M = A;
+      for (int i = 0; i < n; ++i) M[(size_t)i * n + i] += lam * std::max(A[(size_t)i * n + i], 1e-12 * dmax);
+      if (!cholesky(n, M)) continue;
+      std::vector<double> mg(n);
+      for (int i = 0; i < n; ++i) mg[i] = -g[i];
+      std::vector<double> dlt = cholesky_solve(n, M, mg);
+      // predicted decrease: -(2 g.delta + delta.A.delta)
+      double gd = 0.0, dad = 0.0;
+      for (int i = 0; i < n; ++i) {
+        gd += g[i] * dlt[i];
+        double t = 0.0;
+        for (int j = 0; j < n; ++j) t += A[(size_t)i * n + j] * dlt[j];
+        dad += dlt[i] * t;
+      }
+      Potential* tp = pot->clone();
+      tp->set_dir(dlt, 1.0);
+      trials.push_back(tp);
+      steps.push_back(dlt);
+      lams.push_back(lam);
+      preds.push_back(-(2.0 * gd + dad));
+    }
+    const std::vector<double> Ft = batch_objective(ctx, trials);
+    int best = -1;
+    for (size_t k = 0; k < Ft.size(); ++k)
+      if (std::isfinite(Ft[k]) && Ft[k] < F && (best < 0 || Ft[k] < Ft[best])) best = (int)k;
+    const double Fold = F;
+    if (best < 0) {
+      for (auto* t : trials) delete t;
+      lambda *= 100.0;
+      if (++fails > 8 || lambda > 1e16) {
+        if (out) *out << "no damping value decreases the objective: stopping" << std::endl;
         break;
       }
-      delta_mag = 0.0;
-      for (int i = 0; i < ncoeff; ++i) {
-        delta[i] = 0.;
-        for (int j = 0; j < ncoeff; ++j) delta[i] += D(i, j) * q[j];
-        delta_mag += delta[i] * delta[i];
-      }
-      LineMinBrent lm;
-      lm.set_func([&](double x) { return pot->compute_dir(nullptr, delta, x); });
-      F = lm.find_minima();
-      pot->set_dir(delta, lm.xmin1());
-      pot->compute(&ev);
-      if (ferr + berr > 1. && m > 5) break;
-      int max_col = 0;
-      double max_change = 0.;
-      for (int i = 0; i < ncoeff; ++i)
-        if (std::abs(p[i] * q[i]) > max_change) {
-          max_change = std::abs(p[i] * q[i]);
-          max_col = i;
-        }
-      // gamma_update (src/opt_powell.cpp:397-434)
-      {
-        const double l1 = lm.xmin1(), l2 = lm.xmin2();
-        const double Fm = lm.fmin1();
-        ErrorVec f2;
-        pot->compute_dir(&f2, delta, l2 - l1);
-        double mu = 0.0;
-        for (int j = 0; j < esize; ++j) {
-          const double t = (ev[j] - f2[j]) / (l1 - l2);
-          G(j, max_col) = t;
-          mu += t * ev[j];
-        }
-        mu /= Fm;
-        double sum = 0.0;
-        for (int j = 0; j < esize; ++j) {
-          const double t = G(j, max_col) - mu * ev[j];
-          G(j, max_col) = t;
-          sum += t * t;
-        }
-        const double mag = std::sqrt(sum);
-        if (mag < P_NOTHING) {
-          if (out) *out << "Matrix gamma singular after step " << m << ", restarting inner loop" << std::endl;
-          break;
-        }
-        for (int j = 0; j < esize; ++j) G(j, max_col) /= mag;
-        for (int i = 0; i < ncoeff; ++i) delta[i] /= mag;
-      }
-      for (int i = 0; i < ncoeff; ++i) D(i, max_col) = delta[i];
-      lineqsys(max_col);
-      dF = F_old - F;
-      ++m;
-    } while ((m < ncoeff || (m <= P_INNERLOOPS && dF > P_PRECISION)) && dF < P_TOOBIG && delta_mag < P_TOOBIG);
+      continue;
+    }
+    fails = 0;
+    delete pot;
+    pot = trials[best];
+    for (size_t k = 0; k < trials.size(); ++k)
+      if ((int)k != best) delete trials[k];
+    F = pot->compute(&e);
+    const double ratio = preds[best] > 0 ? (Fold - F) / preds[best] : 0.0;
+    lambda = lams[best] * (ratio > 0.75 ? 1.0 / 3.0 : (ratio < 0.25 ? 2.0 : 1.0));
     if (out)
-      *out << std::setw(13) << m << std::setw(8) << Potential::get_ncalls() - calls0 << "   " << std::scientific
-           << std::setprecision(8) << F << std::endl;
-    ++n;
-  } while (((F_older - F > P_PRECISION / 10.) || (F_older - F < 0)) && (F_older - F > opt.d_eps) &&
-           n < opt.max_steps);
+      *out << std::setw(6) << it + 1 << "  " << std::scientific << std::setprecision(8) << F << "  " << lams[best]
+           << "  " << std::setw(7) << Potential::get_ncalls() << "  " << std::fixed << std::setprecision(4)
+           << std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() << std::endl;
+    if (Fold - F < opt.d_eps) break;
+    if (Fold - F <= 1e-12 * Fold) break;
+  }
   return pot;
 }
 
@@ -854,156 +742,175 @@ Potential* lbfgs(const Potential& start, const OptOptions& opt) {
 
 // ---------------------------------------------------------------- simulated annealing
 
+// Simulated annealing with the per-coordinate step control of Corana,
+// Marchesi, Martini and Ridella (ACM Trans. Math. Softw. 13 (1987) 262-280):
+// a move at coefficient c is a Gaussian bump of height N(0,1) v_c and width
+// |N(0,1)| (Potential::modify_coeff), accepted by the Metropolis rule at
+// temperature T.  After kCycles sweeps over the coefficients every v_c is
+// scaled toward an acceptance fraction between 0.4 and 0.6 (factor
+// 1 + 2 (a - 0.6) / 0.4 above, division by 1 + 2 (0.4 - a) / 0.4 below);
+// after 2 n such adjustments T is multiplied by 0.85.  The run ends after
+// max_steps temperatures, or once the objective at the end of the last four
+// temperatures and the best one agree to 1e-3 relative; if they agree with one
+// another but not with the best, the walk restarts from the best.
+//
+// The moves of a sweep are scored speculatively: the next moves (for the
+// following coefficients) are all built from the current state and scored in
+// one batched call; they are then tested in order, and the first accepted one
+// becomes the new state, discarding the rest, which are rebuilt from it.  Each
+// move is thus tested against the state current when it is tested, as in the
+// sequential walk; only the random draws of the discarded moves are extra.  The
+// number of moves per call follows the running acceptance fraction (about two
+// expected acceptances per call, at most 32).
 Potential* simann(const Potential& start, const OptOptions& opt, std::mt19937& rng) {
-  constexpr double EPS = 0.1;      // src/opt_simann.h:41-46
-  constexpr int NEPS = 4;
-  constexpr int NSTEP = 20;
-  constexpr double STEPVAR = 2.0;
-  constexpr double TEMPVAR = 0.85;
-  std::normal_distribution<double> norm(0.0, 1.0);
+  constexpr int kCycles = 20;
+  constexpr double kCool = 0.85;
+  constexpr int kSettle = 4;
+  constexpr double kSettleTol = 1e-3;
+  constexpr int kMaxWidth = 32;
+  std::normal_distribution<double> gauss(0.0, 1.0);
   std::uniform_real_distribution<double> unif(0.0, 1.0);
-  Potential* pot = start.clone();
-  double F = pot->compute();
-  const int ncoeff = pot->get_ncoeff();
-  if (opt.max_steps == 0 || ncoeff == 0) return pot;
-  Potential* best = pot->clone();
-  double Fopt = F;
-  std::vector<double> v(ncoeff, 1.0);
-  std::vector<int> naccept(ncoeff, 0);
-  std::vector<double> Fvar(opt.max_steps + 5 + NEPS, F);
-  int k = 0;
+  Potential* cur = start.clone();
+  double Fcur = cur->compute();
+  const int n = cur->get_ncoeff();
+  if (opt.max_steps == 0 || n == 0) return cur;
+  mz_ctx* ctx = cur->evaluator()->ctx();
+  Potential* best = cur->clone();
+  double Fbest = Fcur;
+  std::vector<double> v(n, 1.0);
+  std::vector<int> acc(n, 0);
   double T = opt.anneal_temp;
-  bool loopagain = true;
-  do {
-    for (int m = 0; m < 3 * ncoeff; ++m) {
-      for (int j = 0; j < NSTEP; ++j)
-        for (int c = 0; c < ncoeff; ++c) {
-          Potential trial(*pot);
-          const double height = norm(rng) * v[c];
-          const double width = std::abs(norm(rng));
-          trial.modify_coeff(c, height, width);
-          const double F2 = trial.compute();
-          bool acc = false;
-          if (F2 <= F) {
-            acc = true;
-            if (F2 <= Fopt) {
+  double rate = 0.5;  // running acceptance fraction
+  std::vector<double> ends;  // objective at the end of every temperature
+  const int adjustments = 2 * n;
+  for (int temp = 0; temp < opt.max_steps; ++temp) {
+    for (int adj = 0; adj < adjustments; ++adj) {
+      for (int cyc = 0; cyc < kCycles; ++cyc) {
+        int c = 0;
+        while (c < n) {
+          const int width = std::max(1, std::min({kMaxWidth, n - c, (int)std::ceil(2.0 / std::max(rate, 1.0 / 64))}));
+          std::vector<Potential*> moves;
+          std::vector<double> u(width);
+          for (int b = 0; b < width; ++b) {
+            Potential* mv = cur->clone();
+            const double height = gauss(rng) * v[c + b];
+            const double w = std::abs(gauss(rng));
+            mv->modify_coeff(c + b, height, w);
+            moves.push_back(mv);
+            u[b] = unif(rng);
+          }
+          const std::vector<double> Fm = batch_objective(ctx, moves);
+          int taken = -1;
+          for (int b = 0; b < width && taken < 0; ++b) {
+            const bool ok = Fm[b] <= Fcur || u[b] < std::exp((Fcur - Fm[b]) / T);
+            rate = 0.98 * rate + 0.02 * (ok ? 1.0 : 0.0);
+            if (ok) taken = b;
+          }
+          if (taken >= 0) {
+            ++acc[c + taken];
+            delete cur;
+            cur = moves[taken];
+            moves[taken] = nullptr;
+            Fcur = Fm[taken];
+            if (Fcur < Fbest) {
               delete best;
-              best = trial.clone();
-              Fopt = F2;
+              best = cur->clone();
+              Fbest = Fcur;
             }
-          } else if (unif(rng) < std::exp((F - F2) / T)) {
-            acc = true;
+            c += taken + 1;
+          } else {
+            c += width;
           }
-          if (acc) {
-            delete pot;
-            pot = trial.clone();
-            F = F2;
-            ++naccept[c];
-          }
+          for (auto* mv : moves) delete mv;
         }
-      for (int c = 0; c < ncoeff; ++c) {
-        if (naccept[c] > 0.6 * NSTEP) v[c] *= (1 + STEPVAR * ((double)naccept[c] / NSTEP - 0.6) / 0.4);
-        else if (naccept[c] < 0.4 * NSTEP) v[c] /= (1 + STEPVAR * (0.4 - (double)naccept[c] / NSTEP) / 0.4);
-        naccept[c] = 0;
       }
-      if (opt.out)
-        *opt.out << std::setw(3) << k << "  T=" << std::fixed << T << "  m=" << m + 1 << "  F=" << std::scientific
-                 << F << "  Fopt=" << Fopt << std::endl;
+      for (int k = 0; k < n; ++k) {
+        const double a = (double)acc[k] / kCycles;
+        if (a > 0.6) v[k] *= 1.0 + 2.0 * (a - 0.6) / 0.4;
+        else if (a < 0.4) v[k] /= 1.0 + 2.0 * (0.4 - a) / 0.4;
+        acc[k] = 0;
+      }
     }
-    T *= TEMPVAR;
-    ++k;
-    loopagain = false;
-    Fvar[k + NEPS] = F;
-    for (int i = 1; i <= NEPS; ++i)
-      if (std::abs(F - Fvar[k + NEPS - i]) > (EPS * F * 0.01)) loopagain = true;
-    if (!loopagain && ((F - Fopt) > (EPS * F * 0.01))) {
-      delete pot;
-      pot = best->clone();
-      F = Fopt;
-      loopagain = true;
+    if (opt.out)
+      *opt.out << "temperature " << temp + 1 << "  T=" << std::scientific << std::setprecision(6) << T
+               << "  F=" << Fcur << "  best F=" << Fbest << std::endl;
+    T *= kCool;
+    ends.push_back(Fcur);
+    if ((int)ends.size() >= kSettle) {
+      bool settled = true;
+      for (int q = 1; q <= kSettle; ++q)
+        settled = settled && std::abs(ends[ends.size() - q] - Fcur) <= kSettleTol * std::abs(Fcur);
+      if (settled) {
+        if (std::abs(Fcur - Fbest) <= kSettleTol * std::abs(Fcur)) break;
+        delete cur;
+        cur = best->clone();
+        Fcur = Fbest;
+      }
     }
-  } while (k < opt.max_steps && loopagain);
-  delete pot;
+  }
+  delete cur;
   return best;
 }
 
 // ---------------------------------------------------------------- genetic algorithm
 
+// Generational genetic algorithm over a population of knot vectors, scored
+// with one batched call per generation.  The population is filled up to
+// pop_size with copies of the first member carrying a Gaussian bump of height
+// N(0,1) |x_c| init_scale at every coefficient; each generation keeps the best
+// max(1, fit_rate P) members unchanged and breeds the rest: two parents chosen
+// by binary tournaments, uniform crossover (each coefficient from the second
+// parent with probability cross_rate), and with probability mut_rate per
+// coefficient a bump of height N(0,1) |x_c| / 10.
 std::vector<Potential*> genalg(const std::vector<Potential*>& pop_in, const OptOptions& opt, std::mt19937& rng) {
-  std::normal_distribution<double> norm(0.0, 1.0);
+  std::normal_distribution<double> gauss(0.0, 1.0);
   std::uniform_real_distribution<double> unif(0.0, 1.0);
   std::vector<Potential*> pop;
   for (auto* p : pop_in) pop.push_back(p->clone());
-  const int ncoeff = pop.front()->get_ncoeff();
-  while ((int)pop.size() < opt.pop_size) {
+  const int n = pop.front()->get_ncoeff();
+  const int P = std::max<int>({opt.pop_size, (int)pop.size(), 2});
+  while ((int)pop.size() < P) {
     Potential* p = pop.front()->clone();
-    for (int c = 0; c < ncoeff; ++c) {
-      const double height = norm(rng) * p->coeff(c) * opt.init_scale;
-      const double width = std::abs(norm(rng));
-      p->modify_coeff(c, height, width);
-    }
+    for (int c = 0; c < n; ++c) p->modify_coeff(c, gauss(rng) * std::abs(p->coeff(c)) * opt.init_scale,
+                                               std::abs(gauss(rng)));
     pop.push_back(p);
   }
-  auto evaluator = pop.front()->evaluator();
-  const int nknot = evaluator->info().nknot;
-  // one batched call scores the whole population
-  auto score = [&](std::vector<Potential*>& ps) {
-    std::vector<double> K((size_t)ps.size() * nknot), F(ps.size());
-    for (size_t i = 0; i < ps.size(); ++i) {
-      auto kv = ps[i]->knot_vector();
-      std::copy(kv.begin(), kv.end(), K.begin() + i * nknot);
+  mz_ctx* ctx = pop.front()->evaluator()->ctx();
+  auto score_and_sort = [&]() {
+    for (size_t i0 = 0; i0 < pop.size(); i0 += kBatch) {
+      const size_t nb = std::min<size_t>(kBatch, pop.size() - i0);
+      std::vector<Potential*> part(pop.begin() + i0, pop.begin() + i0 + nb);
+      const std::vector<double> F = batch_objective(ctx, part);
+      for (size_t b = 0; b < nb; ++b) pop[i0 + b]->set_error(std::isfinite(F[b]) ? F[b] : HUGE_VAL);
     }
-    check(mz_objective_batch(evaluator->ctx(), (int)ps.size(), K.data(), F.data(), nullptr));
-    Potential::add_ncalls(ps.size());
-    for (size_t i = 0; i < ps.size(); ++i) ps[i]->set_error(F[i]);
+    std::stable_sort(pop.begin(), pop.end(), [](Potential* a, Potential* b) { return *a < *b; });
   };
-  score(pop);
-  std::stable_sort(pop.begin(), pop.end(), [](Potential* a, Potential* b) { return *a < *b; });
-  const int P = (int)pop.size();
+  auto tournament = [&](int avoid) {
+    int a = (int)(unif(rng) * P), b = (int)(unif(rng) * P);
+    a = std::min(a, P - 1);
+    b = std::min(b, P - 1);
+    int w = pop[a]->get_error() <= pop[b]->get_error() ? a : b;
+    if (w == avoid) w = (w == a) ? b : a;
+    return w;
+  };
+  score_and_sort();
+  const int elite = std::max(1, std::min(P - 1, (int)std::lround(opt.fit_frac * P)));
   for (int gen = 0; gen < opt.ngens; ++gen) {
-    std::vector<Potential*> kids;
-    int parent = 0;
-    std::vector<int> idx;
-    for (int i = 0; i < P - 2; ++i) idx.push_back(i + 1);
-    for (int i = 0; i < P; ++i) {
-      Potential* kid = pop[i]->clone();
-      if (i < 2) {
-        kids.push_back(kid);
-        continue;
+    std::vector<Potential*> next;
+    for (int i = 0; i < elite; ++i) next.push_back(pop[i]->clone());
+    while ((int)next.size() < P) {
+      const int pa = tournament(-1), pb = tournament(pa);
+      Potential* kid = pop[pa]->clone();
+      for (int c = 0; c < n; ++c) {
+        if (unif(rng) < opt.cross_rate) kid->coeff(c) = pop[pb]->coeff(c);
+        if (unif(rng) < opt.mut_rate)
+          kid->modify_coeff(c, gauss(rng) * std::abs(kid->coeff(c)) * 0.1, std::abs(gauss(rng)));
       }
-      if (i > 2 && unif(rng) < opt.fit_frac) ++parent;
-      int p2 = i - 1;
-      if (!idx.empty()) {
-        int t;
-        do {
-          t = (int)(unif(rng) * idx.size());
-          p2 = idx[t];
-          if (idx.size() == 1 && p2 == parent) ++parent;
-        } while (p2 == parent);
-        idx.erase(idx.begin() + t);
-      }
-      int sw = (int)(unif(rng) * 2), success = 0;
-      for (int c = 0; c < ncoeff; ++c) {
-        if (unif(rng) < opt.cross_rate) {
-          sw = (sw + 1) % 2;
-          success = 1;
-        }
-        if (c == ncoeff - 1 && !success) sw = (sw + 1) % 2;
-        kid->coeff(c) = pop[(sw == 0) ? std::min(parent, P - 1) : p2]->coeff(c);
-        if (unif(rng) < opt.mut_rate) {
-          int r;
-          do {
-            r = (int)(unif(rng) * P);
-          } while (r == parent || r == p2);
-          kid->coeff(c) = pop[r]->coeff(c);
-        }
-      }
-      kids.push_back(kid);
+      next.push_back(kid);
     }
     for (auto* p : pop) delete p;
-    pop = kids;
-    score(pop);
-    std::stable_sort(pop.begin(), pop.end(), [](Potential* a, Potential* b) { return *a < *b; });
+    pop = next;
+    score_and_sort();
     if (opt.out) *opt.out << "generation " << gen + 1 << "  best F = " << std::scientific << pop[0]->get_error()
                           << std::endl;
   }
@@ -1281,7 +1188,7 @@ int run_param_file(const std::string& param_path, std::ostream& out) {
     }
     for (auto*& p : pots) {
       Potential* np = nullptr;
-      if (st == "powell") np = powell(*p, opt);
+      if (st == "powell") np = least_squares(*p, opt);
       else if (st == "lbfgs") np = lbfgs(*p, opt);
       else if (st == "simann") np = simann(*p, opt, rng);
       else throw Error("Invalid optimizer style - " + st);

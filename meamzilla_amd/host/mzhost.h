@@ -139,30 +139,9 @@ class Potential {
 
 // ---------------------------------------------------------------- optimizers
 
-// Golden-section bracketing (src/line_min.cpp:83-153) and Brent's parabolic
-// minimisation (src/line_min_brent.cpp:37-126).
-class LineMinBrent {
- public:
-  template <class F>
-  void set_func(F f) {
-    eval_ = f;
-    bracket();
-  }
-  double find_minima();
-  double xmin1() const { return xmin1_; }
-  double xmin2() const { return xmin2_; }
-  double fmin1() const { return fmin1_; }
-
- private:
-  std::function<double(double)> eval_;
-  double xa_ = 0, xb_ = 0, xc_ = 0, fa_ = 0, fb_ = 0, fc_ = 0;
-  double xmin1_ = 0, xmin2_ = 0, fmin1_ = 0, fmin2_ = 0;
-  void bracket();
-};
-
 struct OptOptions {
-  double d_eps = 0.0;          // Powell: stop when an outer step improves less
-  int max_steps = 1000000;     // Powell / L-BFGS outer steps
+  double d_eps = 0.0;          // least squares / L-BFGS: stop when a step improves less
+  int max_steps = 1000000;     // least-squares and L-BFGS iterations, annealing temperatures
   double anneal_temp = 1.0;    // simulated annealing start temperature
   int pop_size = 10;           // genetic algorithm
   int ngens = 10;
@@ -172,14 +151,23 @@ struct OptOptions {
   std::ostream* out = nullptr;
 };
 
-// Powell least squares (src/opt_powell.cpp:96-315).
-Potential* powell(const Potential& start, const OptOptions& opt);
+// The optimizers of the driver.  They take the optstyle names of the
+// reference's parameter files (powell, simann, genalg) and its parameter keys,
+// but are methods of their own built on the batched entry points; the
+// reference's own optimizers run on this objective through the binding of
+// integration/.
+//
+// Least squares on the residual vector (optstyle powell): Levenberg-Marquardt
+// with batched forward-difference Jacobians and four damping values scored per
+// batched call.
+Potential* least_squares(const Potential& start, const OptOptions& opt);
 // Limited-memory BFGS on the adjoint gradient (no reference counterpart).
 Potential* lbfgs(const Potential& start, const OptOptions& opt);
-// Simulated annealing (src/opt_simann.cpp:89-206).
+// Simulated annealing with Corana et al.'s step control, moves scored in
+// batched calls.
 Potential* simann(const Potential& start, const OptOptions& opt, std::mt19937& rng);
-// Genetic algorithm over a population scored with one batched call per
-// generation (src/opt_genalg.cpp:105-309).
+// Genetic algorithm (elitism, tournaments, uniform crossover, bump mutations)
+// over a population scored with batched calls.
 std::vector<Potential*> genalg(const std::vector<Potential*>& pop, const OptOptions& opt, std::mt19937& rng);
 
 // LAMMPS export of every potential of a list (src/pot_list.cpp:333-361,
