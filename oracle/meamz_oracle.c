@@ -949,6 +949,12 @@ static const long long* g_sort_key;
 static int cmp_trip_desc_g(const void* x, const void* y) { return cmp_trip_desc(x, y, (void*)g_sort_key); }
 
 double orc_eval_mt(orc_ctx* c, const double* knots, int nthreads) {
+  return orc_eval_mt_out(c, knots, nthreads, NULL, NULL, NULL, NULL);
+}
+
+/* orc_eval_mt with the outputs of orc_eval (any may be NULL). */
+double orc_eval_mt_out(orc_ctx* c, const double* knots, int nthreads, double* errvec, double* cell_energy,
+                       double* cell_stress, double* forces) {
   if (nthreads < 1) nthreads = 1;
   refresh_all(c, knots);
   /* greedy balance by per-atom work (triplets + pairs), largest first */
@@ -989,7 +995,7 @@ double orc_eval_mt(orc_ctx* c, const double* knots, int nthreads) {
     for (int k = 0; k < 6 * c->ncells; ++k) r->stress[k] += o->stress[k];
     for (int k = 0; k < 3 * c->natoms; ++k) r->force[k] += o->force[k];
   }
-  double F = finish(c, r, NULL, NULL, NULL, NULL);
+  double F = finish(c, r, errvec, cell_energy, cell_stress, forces);
   for (int t = 0; t < nthreads; ++t) { accum_free(&ws[t].acc); free(lists[t]); }
   free(ws); free(th); free(lists); free(fill); free(cnt); free(load); free(owner); free(order); free(key);
   return F;

@@ -81,6 +81,9 @@ double orc_eval(orc_ctx* ctx, const double* knots, double* errvec, double* cell_
  * count and one sum-reduction at the end (the reference's MPI decomposition,
  * src/atom_vec.cpp:41-97 and src/potential.cpp:446, emulated with threads). */
 double orc_eval_mt(orc_ctx* ctx, const double* knots, int nthreads);
+/* orc_eval_mt with the outputs of orc_eval (each may be NULL). */
+double orc_eval_mt_out(orc_ctx* ctx, const double* knots, int nthreads, double* errvec, double* cell_energy,
+                       double* cell_stress, double* forces);
 
 /* Per-atom densities without clamping (PotMEAMSpline::fast_compute_densities). */
 void orc_densities(orc_ctx* ctx, const double* knots, double* dens);

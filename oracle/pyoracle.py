@@ -42,6 +42,8 @@ def lib():
         L.orc_eval.argtypes = [C.c_void_p, dp, dp, dp, dp, dp]
         L.orc_eval_mt.restype = C.c_double
         L.orc_eval_mt.argtypes = [C.c_void_p, dp, C.c_int]
+        L.orc_eval_mt_out.restype = C.c_double
+        L.orc_eval_mt_out.argtypes = [C.c_void_p, dp, C.c_int, dp, dp, dp, dp]
         L.orc_densities.argtypes = [C.c_void_p, dp, dp]
         ip = C.POINTER(C.c_int)
         L.orc_spline_refresh.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, ip, dp, dp, dp]
@@ -132,6 +134,16 @@ class Oracle:
     def eval_mt(self, knots: Optional[np.ndarray] = None, nthreads: int = 1) -> float:
         kv = np.ascontiguousarray(self.pot.knot_vector() if knots is None else knots, dtype=np.float64)
         return lib().orc_eval_mt(self._ctx, _dp(kv), int(nthreads))
+
+    def eval_full_mt(self, knots: Optional[np.ndarray] = None, nthreads: int = 16) -> dict:
+        """eval() on nthreads threads (atoms balanced by work, one final sum)."""
+        kv = np.ascontiguousarray(self.pot.knot_vector() if knots is None else knots, dtype=np.float64)
+        ev = np.zeros(self.nerr)
+        ce = np.zeros(self.ncells)
+        cs = np.zeros((self.ncells, 6))
+        fr = np.zeros((self.natoms, 3))
+        F = lib().orc_eval_mt_out(self._ctx, _dp(kv), int(nthreads), _dp(ev), _dp(ce), _dp(cs), _dp(fr))
+        return dict(F=F, errvec=ev, cell_energy=ce, cell_stress=cs, forces=fr)
 
     def densities(self, knots: Optional[np.ndarray] = None) -> np.ndarray:
         kv = np.ascontiguousarray(self.pot.knot_vector() if knots is None else knots, dtype=np.float64)
