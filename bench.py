@@ -18,6 +18,10 @@ One step = one call of the selected mode:
   cd      : objective + central-difference gradient of one knot vector
             (2*ncoeff+1 plain evaluations, the per-vector path an optimizer
             without an analytic gradient uses)
+  plain   : objective of one knot vector (mz_objective through its captured
+            graph, the latency an annealing trial or a line-search point
+            pays); the timed steps replay the graph with timing off, and the
+            kernel durations come from a separate pass of the same steps
 `value` is objective+gradient evaluations per second of the selected mode.
 The plain objective rate, the plain+CD gradient rate and the single-vector
 adjoint rate are measured beside it, so the line also carries the
@@ -319,26 +323,35 @@ def measure(wl: int, steps: int, warmup: int, mode: str, cand: int, side: bool, 
 
     # timed region: K steps of the mode with the HIP events around every
     # kernel on the library's stream
-    step, per_step = {"batch": (batch, cand), "adjoint": (adjoint, 1), "cd": (cd, 1)}[mode]
-    obj.enable_timing(True)
-    for _ in range(warmup):
-        step()
+    step, per_step = {"batch": (batch, cand), "adjoint": (adjoint, 1), "cd": (cd, 1), "plain": (plain, 1)}[mode]
     tm = {"atom_ms": [], "adjoint_ms": [], "resid_ms": [], "refresh_ms": [], "total_ms": [], "launches": []}
-    ts = time.perf_counter()
-    for _ in range(steps):
-        step()  # every call returns after its stream has drained
-        t = obj.last_timing()
-        for k in tm:
-            tm[k].append(getattr(t, k))
-    dt = (time.perf_counter() - ts) / steps
-    obj.enable_timing(False)
+
+    def timed_steps(record):
+        obj.enable_timing(record)
+        for _ in range(warmup):
+            step()
+        ts = time.perf_counter()
+        for _ in range(steps):
+            step()  # every call returns after its stream has drained
+            if record:
+                t = obj.last_timing()
+                for k in tm:
+                    tm[k].append(getattr(t, k))
+        obj.enable_timing(False)
+        return (time.perf_counter() - ts) / steps
+
+    if mode == "plain":
+        dt = timed_steps(False)  # the captured graph, as an optimizer replays it
+        timed_steps(True)
+    else:
+        dt = timed_steps(True)
     value = per_step / dt
     log(f"{mode}: {value:.1f} evals/s over {steps} steps")
     ms = {k: float(np.mean(v)) for k, v in tm.items() if k != "launches"}  # per step, summed over launches
-    launches = int(round(float(np.mean(tm["launches"])))) if mode != "cd" else 1
+    launches = int(round(float(np.mean(tm["launches"])))) if mode not in ("cd", "plain") else 1
     cand_step = per_step if mode == "batch" else 1
     cand_launch = cand_step / launches
-    grad = mode != "cd"
+    grad = mode not in ("cd", "plain")
 
     fwd_flop = (FWD_FLOP_PER_TRIPLET * T + FWD_FLOP_PER_PAIR * P) * cand_step
     adj_flop = (ADJ_FLOP_PER_TRIPLET * T + ADJ_FLOP_PER_PAIR * P) * cand_step
@@ -350,7 +363,7 @@ def measure(wl: int, steps: int, warmup: int, mode: str, cand: int, side: bool, 
                     * N) * cand_step)
     adj_bytes = ((ADJ_SHARED_BYTES_PER_PAIR * P + ADJ_SHARED_BYTES_PER_ATOM * N) * launches +
                  (ADJ_BYTES_PER_PAIR * P + ADJ_BYTES_PER_ATOM * N) * cand_step)
-    if mode == "cd":
+    if not grad:
         dom = {"kernel": "k_atoms", "ms": ms["atom_ms"], "flop": fwd_flop,
                "work": f"({FWD_FLOP_PER_TRIPLET} FLOP/triplet x {T} + {FWD_FLOP_PER_PAIR} FLOP/pair x {P})"}
     else:
@@ -416,7 +429,7 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", type=int, default=4)
-    ap.add_argument("--mode", choices=["batch", "adjoint", "cd"], default="batch")
+    ap.add_argument("--mode", choices=["batch", "adjoint", "cd", "plain"], default="batch")
     ap.add_argument("--cand", type=int, default=64, help="knot vectors per batched call")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
