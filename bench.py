@@ -285,9 +285,18 @@ def measure(wl: int, steps: int, warmup: int, mode: str, cand: int, side: bool, 
     gen.add_targets(w, gpu_evaluator)
     log(f"workload {w.name}: {w.db.ncells} cells, {w.db.total_atoms} atoms, built in {time.time() - t0:.1f}s")
     obj = Objective(w.pot, w.db)
+    tune_s = 0.0
     if tile:
-        a, p, b = (int(x) for x in tile.split(","))
-        obj.set_tile(a, p, b)
+        v = [int(x) for x in tile.split(",")]
+        obj.set_tile(v[0], v[1], v[2], v[3] if len(v) > 3 else 2)
+    else:
+        # the work decomposition the library's autotuner picks for this
+        # database (once per database, like the list build; not timed)
+        t0 = time.time()
+        obj.autotune(ncand=min(cand, 32), reps=2)
+        tune_s = time.time() - t0
+        t = obj.get_tile()
+        log(f"autotuned tile {t.max_atoms},{t.max_pairs},{t.block},{t.cand_tile} in {tune_s:.1f}s")
     info = obj.info
     P, T, N, Ccells = info.npairs, info.ntriplets, info.natoms, info.ncells
     log(f"pairs {P} triplets {T} chunks {info.nchunks}")
@@ -385,7 +394,8 @@ def measure(wl: int, steps: int, warmup: int, mode: str, cand: int, side: bool, 
                     f"cutoff {w.cutoff:g} A",
         "name": w.name, "cells": Ccells, "atoms": N, "pairs": P, "triplets": T, "ncoeff": ncoeff, "mode": mode,
         "candidates_per_step": per_step, "launches_per_step": launches, "candidates_per_launch": cand_launch,
-        "tile": [tile_now.max_atoms, tile_now.max_pairs, tile_now.block],
+        "tile": [tile_now.max_atoms, tile_now.max_pairs, tile_now.block, tile_now.cand_tile],
+        "autotune_s": tune_s,
         "value": value, "ms_per_step": dt * 1e3,
         "objective_evals_per_s": _num(1.0 / plain_dt),
         "objective_grad_cd_evals_per_s": _num(1.0 / cd_dt),
@@ -433,7 +443,9 @@ def main():
     ap.add_argument("--cand", type=int, default=64, help="knot vectors per batched call")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--tile", type=str, default="", help="max_atoms,max_pairs,block")
+    ap.add_argument("--tile", type=str, default="",
+                    help="max_atoms,max_pairs,block[,candidates per workgroup] (default: the tile mz_autotune "
+                         "picks for the database)")
     ap.add_argument("--no-side", action="store_true",
                     help="skip the side rates (plain, CD, single adjoint) and the extra workload; for profiler runs")
     ap.add_argument("--no-extra", action="store_true", help="skip the one-element workload figures")
@@ -471,7 +483,7 @@ def main():
         "dtype": "f64",
         "data": "synthetic (seeded geometry, knots and noisy targets)",
         "config": {k: res[k] for k in ("workload", "cells", "atoms", "pairs", "triplets", "ncoeff", "mode",
-                                       "candidates_per_step", "launches_per_step", "tile")},
+                                       "candidates_per_step", "launches_per_step", "tile", "autotune_s")},
     }
     for k in ("objective_evals_per_s", "objective_grad_cd_evals_per_s", "objective_grad_adjoint_evals_per_s",
               "batched_vs_plain_cd", "pairs_per_s", "triplets_per_s", "kernel_ms_per_step", "roofline",

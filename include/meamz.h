@@ -173,8 +173,10 @@ int mz_objective_batch(mz_ctx* ctx, int ncand, const double* knots, double* F, d
 int mz_residuals_batch(mz_ctx* ctx, int ncand, const double* knots, double* F, double* errvec);
 
 /* Work-decomposition tile: atoms per chunk limit, pairs per chunk limit and
- * candidates per block.  mz_autotune times candidate tiles on the current
- * database and keeps the fastest; mz_get_tile reports the tile in use. */
+ * candidates per block.  mz_autotune times tiles on the current database with
+ * ncand knot vectors per call (a coordinate search: candidates per workgroup,
+ * then threads and pairs per chunk, then atoms per chunk) and keeps the
+ * fastest; mz_get_tile reports the tile in use. */
 typedef struct {
   int max_atoms;        /* atoms per chunk (work unit) */
   int max_pairs;        /* pairs per chunk */
