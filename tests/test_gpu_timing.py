@@ -32,5 +32,10 @@ def test_timed_graphs_report_kernel_times_and_keep_results():
     obj.enable_timing(False)
     assert np.array_equal(f0, f1)
     assert np.array_equal(F0, F1)
-    sc = np.abs(G0) + 1e-12 * np.max(np.abs(G0))
-    assert np.all(np.abs(G1 - G0) <= 1e-13 * sc)
+    # coefficient adjoints are summed with LDS atomics whose order between the
+    # waves of a workgroup can change from run to run
+    for a, b in zip(G0, G1):
+        d = np.abs(b - a)
+        top = np.max(np.abs(a))
+        assert np.max(d) <= 1e-13 * top
+        assert np.all(d <= 1e-12 * np.abs(a) + 1e-14 * top)

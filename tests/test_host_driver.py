@@ -98,3 +98,28 @@ def test_fitrun_lbfgs_and_powell(tmp_path):
         assert r[style]["final_F"] < r[style]["initial_F"], (style, r[style])
         assert r[style]["calls"] > 0 and r[style]["seconds"] >= 0
     assert len(r["lbfgs"]["trajectory"]) >= 1
+
+
+@pytest.mark.gpu
+def test_fitrun_binary_adjoint_workload_lbfgs_200_beside_least_squares(tmp_path):
+    """The 1000-cell binary adjoint workload in full (160 k atoms, 139 free
+    coefficients): 200 L-BFGS iterations on the adjoint gradient beside three
+    least-squares iterations on batched Jacobians, both in the C++ driver on the
+    same files; each lowers the objective by more than half."""
+    import io
+    import json
+    from contextlib import redirect_stdout
+    from meamzilla_amd import fitrun
+    print("fit run: binary adjoint workload, L-BFGS 200 beside least squares", flush=True)
+    buf = io.StringIO()
+    with redirect_stdout(buf):
+        assert fitrun.main(["--workload", "3", "--lbfgs-steps", "200", "--powell-steps", "3",
+                            "--out", str(tmp_path)]) == 0
+    r = json.loads(buf.getvalue().strip().splitlines()[-1])
+    assert r["cells"] == 1000 and r["atoms"] > 150000
+    print({s: (r[s]["initial_F"], r[s]["final_F"], r[s]["calls"], r[s]["seconds"]) for s in ("lbfgs", "powell")},
+          flush=True)
+    for style in ("lbfgs", "powell"):
+        assert r[style]["final_F"] < 0.5 * r[style]["initial_F"], (style, r[style])
+    iters = [t["iter"] for t in r["lbfgs"]["trajectory"] if "iter" in t]
+    assert max(iters) == 200
