@@ -31,8 +31,6 @@
 
 namespace mz {
 
-constexpr int kAdjBlock = 256;   // threads (= pairs) per adjoint workgroup
-constexpr int kAdjAtoms = 64;    // atoms per adjoint chunk
 
 struct AdjArgs {
   PotDesc pot;
@@ -227,16 +225,16 @@ __device__ __forceinline__ void ang_at(const GSpl& G, const double* gt, double c
 // JM: jmeam styles, JM p p q terms: the partner record gains p and a p' per
 // term, the sweep a q evaluation, a push and the q-coefficient adjoints
 // (the g term with psi -> lambda_E, U' -> 1, f -> p).
-template <bool MEAM, bool POINT, bool Q, int GR, bool GP, int JM>
-__global__ void __launch_bounds__(kAdjBlock) k_adjoint(const AdjArgs A) {
+template <bool MEAM, bool POINT, bool Q, int GR, bool GP, int JM, int BLK>
+__global__ void __launch_bounds__(BLK) k_adjoint(const AdjArgs A) {
   extern __shared__ __attribute__((aligned(64))) char dyn[];
   constexpr int NJ = JM > 0 ? JM : 1;
-  __shared__ double pd[kPartnerFields + 2 * JM][kAdjBlock];
-  __shared__ double push[2 + JM][kAdjBlock];
-  __shared__ double psi[kAdjAtoms], Up[kAdjAtoms];
-  static_assert(kAdjAtoms * 4 == kAdjBlock, "phase B takes four lanes per atom");
-  __shared__ int poff[kAdjAtoms + 1];
-  __shared__ int ptyp[GP ? kAdjBlock : 1];  // neighbour type of every pair (GP)
+  constexpr int NA = BLK / 4;  // atoms per chunk: phase B takes four lanes per atom
+  __shared__ double pd[kPartnerFields + 2 * JM][BLK];
+  __shared__ double push[2 + JM][BLK];
+  __shared__ double psi[NA], Up[NA];
+  __shared__ int poff[NA + 1];
+  __shared__ int ptyp[GP ? BLK : 1];  // neighbour type of every pair (GP)
   constexpr int NC = Q ? 6 : 4;  // coefficients per g interval that carry adjoints
   const int cand = blockIdx.y;
   const int tid = threadIdx.x;
@@ -256,14 +254,14 @@ __global__ void __launch_bounds__(kAdjBlock) k_adjoint(const AdjArgs A) {
   double* gt = reinterpret_cast<double*>(dyn + ghb);              // g coefficients, [record][NC]
   double* atab = gt + (MEAM ? A.ngco * NC : 0);                   // coefficient + head adjoints
   double* gacc = atab + A.ntab;                                   // [g record][coef][replica]
-  for (int q = tid; q < ghb / 16; q += kAdjBlock) reinterpret_cast<int4*>(dyn)[q] = reinterpret_cast<const int4*>(tbl)[q];
+  for (int q = tid; q < ghb / 16; q += BLK) reinterpret_cast<int4*>(dyn)[q] = reinterpret_cast<const int4*>(tbl)[q];
   if (MEAM) {
     const SplInterval* src = reinterpret_cast<const SplInterval*>(A.tables + (size_t)cand * A.table_bytes +
                                                                   sizeof(SplHead) * A.pot.nspl);
-    for (int q = tid; q < A.ngco * NC; q += kAdjBlock) gt[q] = src[A.gco0 + q / NC].c[q % NC];
+    for (int q = tid; q < A.ngco * NC; q += BLK) gt[q] = src[A.gco0 + q / NC].c[q % NC];
   }
   const int nacc = A.ntab + (MEAM ? A.ngco * NC * R : 0);
-  for (int q = tid; q < nacc; q += kAdjBlock) atab[q] = 0.0;
+  for (int q = tid; q < nacc; q += BLK) atab[q] = 0.0;
 
   // The grid holds as many workgroups as the GPU keeps resident; workgroup x
   // of a candidate takes chunks x, x + gridDim.x, ... of the decreasing-work
@@ -331,7 +329,7 @@ __global__ void __launch_bounds__(kAdjBlock) k_adjoint(const AdjArgs A) {
     ti = A.types[ig];
     tj = A.types[j];
   }
-  for (int a = tid; a <= ch.na; a += kAdjBlock) poff[a] = A.atom_poff[ch.a0 + a] - ch.p0;
+  for (int a = tid; a <= ch.na; a += BLK) poff[a] = A.atom_poff[ch.a0 + a] - ch.p0;
   __syncthreads();
 
   // ---- phase A: pair duals, S_i, partner data
@@ -487,6 +485,9 @@ __global__ void __launch_bounds__(kAdjBlock) k_adjoint(const AdjArgs A) {
     const GSpl G0 = gspl_of(GP ? GH[gfix] : HD[A.pot.fn_off[4] + ti], A.gco0);
     const double Aj = aj * dfj;
     const double Vjx = invr * (wx - aj * ux), Vjy = invr * (wy - aj * uy), Vjz = invr * (wz - aj * uz);
+    // lane constants: U' A_j, U' f_j, and U' / step of the atom's g
+    const double upAj = up * Aj, upfj = up * fj, upi0 = up * G0.inv;
+    const int kmax0 = (int)G0.kmax;
     double Rj = 0.0;
     GSpl Gq[NJ];  // q (q2) of the atom's type
 #pragma unroll
@@ -500,25 +501,28 @@ __global__ void __launch_bounds__(kAdjBlock) k_adjoint(const AdjArgs A) {
       const double cs = ux * Pa.x + uy * Pa.y + uz * Pa.z;
       const int gi = GP ? gfix + pair_ij(tj, ptyp[k], nt) : gfix;
       const GSpl G = GP ? gspl_of(GH[gi], A.gco0) : G0;
-      const double gx0 = G.x0, ginv = G.inv, gxcut = G.xcut, gxn = G.xn, gkmax = G.kmax;
+      const double ginv = G.inv;
+      const double upi = GP ? up * ginv : upi0;
       const double2* gtg = reinterpret_cast<const double2*>(gt + G.rec * NC);
       double gv, dgt, t;
       int fl;
       bool extrap = false, keep = true;
-      if (POINT && (cs < gx0 || cs > gxn)) {
+      if (POINT && (cs < G.x0 || cs > G.xn)) {
         extrap = true;
-        const bool left = cs < gx0;
-        const double e = cs - (left ? gx0 : gxn);
+        const bool left = cs < G.x0;
+        const double e = cs - (left ? G.x0 : G.xn);
         const double g1 = left ? G.gL : G.gR, g2 = left ? G.g2L : G.g2R;
         gv = (left ? G.y0 : G.yn) + g1 * e + 0.5 * g2 * e * e;
         dgt = (g1 + g2 * e) / ginv;
         t = e;
         fl = left ? 0 : 1;
       } else {
-        const double u = (cs - gx0) * ginv;
-        const double flo = fmin(fmax(floor(u), 0.0), gkmax);
-        t = u - flo;
-        fl = (int)flo;
+        // interval index clamped as an integer (the cosine lies in [-1, 1])
+        const double u = (cs - G.x0) * ginv;
+        const int kmax = GP ? (int)G.kmax : kmax0;
+        fl = (int)floor(u);
+        fl = fl < 0 ? 0 : (fl > kmax ? kmax : fl);
+        t = u - (double)fl;
         const double2* C = gtg + fl * (NC / 2);
         const double2 c01 = C[0], c23 = C[1];
         if (Q) {
@@ -529,7 +533,7 @@ __global__ void __launch_bounds__(kAdjBlock) k_adjoint(const AdjArgs A) {
           gv = c01.x + t * (c01.y + t * (c23.x + t * c23.y));
           dgt = c01.y + t * (2.0 * c23.x + t * 3.0 * c23.y);
         }
-        if (!POINT) keep = cs < gxcut;
+        if (!POINT) keep = cs < G.xcut;
       }
       if (!keep) {
         gv = 0.0;
@@ -537,15 +541,16 @@ __global__ void __launch_bounds__(kAdjBlock) k_adjoint(const AdjArgs A) {
       }
       const double fk = keep ? Pa.w : 0.0;
       const double Ak = keep ? Pb.x : 0.0;
-      const double gp = dgt * ginv;
-      const double Bs = (Vjx * Pa.x + Vjy * Pa.y + Vjz * Pa.z) + (Pb.y * ux + Pb.z * uy + Pb.w * uz);
-      const double X = ps * gv + up * gp * Bs;
+      // B_jk + B_kj, then X = psi g + U' g' (B_jk + B_kj) with g' = dgt / step
+      const double Bs = fma(Vjx, Pa.x, fma(Vjy, Pa.y, fma(Vjz, Pa.z, fma(Pb.y, ux, fma(Pb.z, uy, Pb.w * uz)))));
+      const double X = fma(ps, gv, upi * dgt * Bs);
+      const double ugv = up * gv;
       const double ff = fj * fk;
-      Rj += fk * X + up * gv * Ak;
-      atomicAdd(&push[0][k], fj * X + up * gv * Aj);
-      const double al = ps * ff + up * (Aj * fk + Ak * fj);
-      const double be = up * ff * Bs;
+      Rj = fma(fk, X, fma(ugv, Ak, Rj));
+      atomicAdd(&push[0][k], fma(fj, X, ugv * Aj));
+      const double al = fma(ps, ff, fma(upAj, fk, upfj * Ak));
       if (POINT && extrap) {
+        const double be = up * ff * Bs;
         const bool left = fl == 0;
         const double e = t;
         double* ghead = atab + ncoef * 6 + (A.pot.fn_off[4] + gi) * 6;
@@ -553,17 +558,24 @@ __global__ void __launch_bounds__(kAdjBlock) k_adjoint(const AdjArgs A) {
         atomicAdd(ghead + (left ? 2 : 3), al * e + be);
         if (G.quintic) atomicAdd(ghead + (left ? 4 : 5), 0.5 * al * e * e + be * e);
       } else {
-        const double bt = be * ginv;
-        const double t2 = t * t, t3 = t2 * t;
+        // power-basis moments al t^m + m bt t^(m-1) by Horner steps,
+        // m_k = t m_(k-1) + bt t^(k-1), bt the slope weight per unit t
+        const double bt = upi * (ff * Bs);
         double* cp = gacc + G.rec * NC * R + (tid & (R - 1)) + fl * (NC * R);
+        const double m1 = fma(al, t, bt);
+        const double b2 = bt * t;
+        const double m2 = fma(t, m1, b2);
+        const double b3 = b2 * t;
+        const double m3 = fma(t, m2, b3);
         atomicAdd(cp, al);
-        atomicAdd(cp + R, al * t + bt);
-        atomicAdd(cp + 2 * R, al * t2 + 2.0 * bt * t);
-        atomicAdd(cp + 3 * R, al * t3 + 3.0 * bt * t2);
+        atomicAdd(cp + R, m1);
+        atomicAdd(cp + 2 * R, m2);
+        atomicAdd(cp + 3 * R, m3);
         if (Q) {
-          const double t4 = t3 * t;
-          atomicAdd(cp + 4 * R, al * t4 + 4.0 * bt * t3);
-          atomicAdd(cp + 5 * R, al * t4 * t + 5.0 * bt * t4);
+          const double b4 = b3 * t;
+          const double m4 = fma(t, m3, b4);
+          atomicAdd(cp + 4 * R, m4);
+          atomicAdd(cp + 5 * R, fma(t, m4, b4 * t));
         }
       }
       // jmeam: X_q = lambda_E q + q' (B_jk + B_kj); pair j gains p_k X_q + q A_k,
@@ -615,7 +627,7 @@ __global__ void __launch_bounds__(kAdjBlock) k_adjoint(const AdjArgs A) {
   const int RR = A.rrep_log2 >= 0 ? 1 << A.rrep_log2 : 0;
   const int RS = A.rrep_stride;
   double* rrep = &pd[0][0];
-  for (int q = tid; q < RS * RR; q += kAdjBlock) rrep[q] = 0.0;
+  for (int q = tid; q < RS * RR; q += BLK) rrep[q] = 0.0;
   if (MEAM && has) {
     Phi += push[0][p];
     Phip = Up[ia] * aj * S1;
@@ -650,7 +662,7 @@ __global__ void __launch_bounds__(kAdjBlock) k_adjoint(const AdjArgs A) {
     }
   }
   __syncthreads();
-  for (int q = tid; q < A.nrad * NC * (RR ? 1 : 0); q += kAdjBlock) {
+  for (int q = tid; q < A.nrad * NC * (RR ? 1 : 0); q += BLK) {
     double sacc = 0.0;
     for (int rep = 0; rep < RR; ++rep) sacc += rrep[rep * RS + q];
     atab[(q / NC) * 6 + q % NC] += sacc;
@@ -659,7 +671,7 @@ __global__ void __launch_bounds__(kAdjBlock) k_adjoint(const AdjArgs A) {
   // barrier, which every lane reaches after this reduction.
   }  // chunks
   if (MEAM) {
-    for (int q = tid; q < A.ngco * NC; q += kAdjBlock) {
+    for (int q = tid; q < A.ngco * NC; q += BLK) {
       double sacc = 0.0;
       for (int rep = 0; rep < R; ++rep) sacc += gacc[(size_t)q * R + rep];
       atab[(A.gco0 + q / NC) * 6 + q % NC] += sacc;
@@ -667,7 +679,7 @@ __global__ void __launch_bounds__(kAdjBlock) k_adjoint(const AdjArgs A) {
   }
   __syncthreads();
   double* slot = A.slots + (size_t)cand * A.ntab * gridDim.x + blockIdx.x;
-  for (int q = tid; q < A.ntab; q += kAdjBlock) slot[(size_t)q * gridDim.x] = atab[q];
+  for (int q = tid; q < A.ntab; q += BLK) slot[(size_t)q * gridDim.x] = atab[q];
 }
 
 // Sum of every candidate's per-workgroup partial tables: one 256-lane
@@ -898,7 +910,7 @@ void host_refresh(int basis, int n, double left, double right, int natL, int nat
     if (e_ != hipSuccess) return mz_fail(MZ_ERR_HIP, std::string(#call ": ") + hipGetErrorString(e_)); \
   } while (0)
 
-const void* adjoint_fn(const mz_ctx* c, int grep_log2);
+const void* adjoint_fn(const mz_ctx* c, int grep_log2, int blk);
 
 int setup_adjoint(mz_ctx* c) {
   mz::AdjointExtra& X = c->adjx;
@@ -925,7 +937,56 @@ int setup_adjoint(mz_ctx* c) {
   }
   if (X.d_J.upload(J.data(), J.size()) || X.d_joff.upload(joff.data(), joff.size()))
     return mz_fail(MZ_ERR_NOMEM, "adjoint Jacobian upload failed");
-  // chunks of at most 256 pairs for the adjoint workgroups
+  // LDS: compact g coefficients, coefficient adjoints and the lane replicas
+  // of the g-coefficient adjoints beside the static arrays of the workgroup
+  // size.  Each built (workgroup size, replica count) pair that fits the
+  // device's workgroup LDS is scored by its waves resident per CU times a
+  // replica weight, 1 for 16 or 32 replicas (the lanes of a wave that share a
+  // bank then add into one replica column: no bank conflicts), 3/4 for 8 and
+  // 1/4 for 1; ties go to the smaller workgroup, then to more replicas.  The
+  // weights reproduce the measured order: ternary 2000-cell workload, 32
+  // candidates per launch: 512 x 16 (16 waves) 402 ms, 256 x 8 (12 waves)
+  // 515 ms, 256 x 1 (16 waves) and 512 x 32 (8 waves) 920 ms; one-element
+  // 256-cell workload: 256 x 32 about 8 % ahead of 512 x 32 (both 16 waves).
+  X.ntab = c->ncoef * 6 + c->nspl * 6;
+  X.gco0 = c->meam ? c->coef_off[c->pot.fn_off[4]] : 0;
+  X.ngco = c->meam ? c->ncoef - X.gco0 : 0;
+  const int nc_g = c->pot.quintic ? 6 : 4;
+  const int fixed = (X.ngco * nc_g + X.ntab) * (int)sizeof(double) + c->nspl * (int)sizeof(mz::SplHead);
+  int maxlds = 0, cus = 0;
+  if (hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess || cus < 1)
+    return mz_fail(MZ_ERR_HIP, "device attribute query failed");
+  struct Cfg {
+    int blk, lg;
+  };
+  const Cfg cfgs[] = {{256, 5}, {256, 3}, {256, 0}, {512, 5}, {512, 4}};
+  int best = 0;  // score in quarter waves
+  X.fn = nullptr;
+  for (const Cfg& f : cfgs) {
+    if (!c->meam && f.lg != 0) continue;  // no g spline: no replicas
+    const void* fn = adjoint_fn(c, f.lg, f.blk);
+    if (!fn) continue;
+    hipFuncAttributes fa{};
+    if (hipFuncGetAttributes(&fa, fn) != hipSuccess) continue;
+    const int dyn = fixed + (c->meam ? X.ngco * nc_g * (1 << f.lg) * 8 : 0);
+    if ((long long)fa.sharedSizeBytes + dyn > maxlds) continue;
+    int wpc = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&wpc, fn, f.blk, dyn) != hipSuccess || wpc < 1) continue;
+    const int weight = !c->meam || f.lg >= 4 ? 4 : (f.lg == 3 ? 3 : 1);
+    const int score = wpc * f.blk / 64 * weight;
+    if (score > best) {  // strict: earlier entries win ties
+      best = score;
+      X.fn = fn;
+      X.blk = f.blk;
+      X.grep_log2 = f.lg;
+      X.dyn_bytes = dyn;
+      X.resident = wpc * cus;
+    }
+  }
+  if (!X.fn) return mz_fail(MZ_ERR_CAPACITY, "no adjoint workgroup fits the device's workgroup LDS");
+  // chunks of at most blk pairs and blk / 4 atoms for the adjoint workgroups
+  const int natom_max = X.blk / 4;
   std::vector<mz::Chunk> ch;
   for (int cc = 0; cc < c->ncells; ++cc) {
     int a = c->abase[cc];
@@ -936,11 +997,12 @@ int setup_adjoint(mz_ctx* c) {
       k.a0 = a;
       k.p0 = c->atom_poff[a];
       int na = 0;
-      while (a < aend && na < mz::kAdjAtoms) {
+      while (a < aend && na < natom_max) {
         const int np_a = c->atom_poff[a + 1] - c->atom_poff[a];
-        if (np_a > mz::kAdjBlock)
-          return mz_fail(MZ_ERR_CAPACITY, "atom " + std::to_string(a) + " has more than 256 neighbours");
-        if (c->atom_poff[a + 1] - k.p0 > mz::kAdjBlock) break;
+        if (np_a > X.blk)
+          return mz_fail(MZ_ERR_CAPACITY,
+                         "atom " + std::to_string(a) + " has more than " + std::to_string(X.blk) + " neighbours");
+        if (c->atom_poff[a + 1] - k.p0 > X.blk) break;
         ++a;
         ++na;
       }
@@ -966,40 +1028,12 @@ int setup_adjoint(mz_ctx* c) {
     for (int q = c->atom_poff[a]; q < c->atom_poff[a + 1]; ++q) owner[q] = a;
   if (X.d_owner.upload(owner.data(), owner.size())) return mz_fail(MZ_ERR_NOMEM, "adjoint owner upload failed");
   X.nchunks = (int)sorted.size();
-  X.ntab = c->ncoef * 6 + c->nspl * 6;
-  X.gco0 = c->meam ? c->coef_off[c->pot.fn_off[4]] : 0;
-  X.ngco = c->meam ? c->ncoef - X.gco0 : 0;
-  // LDS: compact g coefficients, coefficient adjoints and the lane replicas
-  // of the g-coefficient adjoints.  The replica count is the largest of 32,
-  // 8, 1 (the counts the kernel is built for) that fits the 64 KB workgroup
-  // limit: replicas (fewer lanes adding into one address) were measured to
-  // matter more than workgroups per CU.
-  const int nc_g = c->pot.quintic ? 6 : 4;
-  const int fixed = (X.ngco * nc_g + X.ntab) * (int)sizeof(double) + c->nspl * (int)sizeof(mz::SplHead);
-  const int jm = c->pot.njm;
-  const int stat = (int)((mz::kPartnerFields + 2 * jm) * sizeof(double) * mz::kAdjBlock +
-                         (2 + jm) * sizeof(double) * mz::kAdjBlock + 2 * sizeof(double) * mz::kAdjAtoms +
-                         sizeof(int) * (mz::kAdjAtoms + 1));
-  X.grep_log2 = -1;
-  for (int lg : {5, 3, 0})  // the replica counts the kernel is built for
-    if (X.grep_log2 < 0 && fixed + (c->meam ? X.ngco * nc_g * (1 << lg) * 8 : 0) + stat <= 64 * 1024)
-      X.grep_log2 = lg;
-  if (X.grep_log2 < 0) return mz_fail(MZ_ERR_CAPACITY, "adjoint workgroup would exceed 64 KB of LDS");
   // Lane replicas of the radial interval records reuse the partner-data area.
   X.nrad = c->meam ? X.gco0 : c->ncoef;
   X.rrep_stride = ((X.nrad * nc_g + 31) & ~31) + 1;
   X.rrep_log2 = -1;
   for (int lg = 4; lg >= 0 && X.rrep_log2 < 0; --lg)
-    if (X.rrep_stride * (1 << lg) <= mz::kPartnerFields * mz::kAdjBlock) X.rrep_log2 = lg;
-  X.dyn_bytes = fixed + (c->meam ? X.ngco * nc_g * (1 << X.grep_log2) * 8 : 0);
-  // resident workgroups of this instantiation on the whole GPU
-  X.fn = adjoint_fn(c, X.grep_log2);
-  int wpc = 0, cus = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&wpc, X.fn, mz::kAdjBlock, X.dyn_bytes) != hipSuccess || wpc < 1)
-    return mz_fail(MZ_ERR_HIP, "occupancy query of the adjoint kernel failed");
-  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess || cus < 1)
-    return mz_fail(MZ_ERR_HIP, "compute-unit count query failed");
-  X.resident = wpc * cus;
+    if (X.rrep_stride * (1 << lg) <= mz::kPartnerFields * X.blk) X.rrep_log2 = lg;
   X.ready = true;
   return MZ_OK;
 }
@@ -1027,30 +1061,42 @@ int ensure_adjoint_buffers(mz_ctx* c, int ncand) {
 
 // The k_adjoint instantiation of a potential: style flags and the lane
 // replica count of the g-coefficient adjoints.
+// Built pairs (workgroup size, g replicas): 512 with 32 or 16, 256 with 32, 8
+// or 1 (the selection list of setup_adjoint).
 template <bool M, bool P, bool Q, bool GP, int JM>
-const void* adjoint_fn_g(int grep_log2) {
-  if (grep_log2 == 5) return reinterpret_cast<const void*>(&mz::k_adjoint<M, P, Q, 32, GP, JM>);
-  if (grep_log2 == 3) return reinterpret_cast<const void*>(&mz::k_adjoint<M, P, Q, 8, GP, JM>);
-  return reinterpret_cast<const void*>(&mz::k_adjoint<M, P, Q, 1, GP, JM>);
+const void* adjoint_fn_g(int grep_log2, int blk) {
+  if (blk == 512) {
+    if (grep_log2 == 5) return reinterpret_cast<const void*>(&mz::k_adjoint<M, P, Q, 32, GP, JM, 512>);
+    if (grep_log2 == 4) return reinterpret_cast<const void*>(&mz::k_adjoint<M, P, Q, 16, GP, JM, 512>);
+    return nullptr;
+  }
+  if (grep_log2 == 5) return reinterpret_cast<const void*>(&mz::k_adjoint<M, P, Q, 32, GP, JM, 256>);
+  if (grep_log2 == 3) return reinterpret_cast<const void*>(&mz::k_adjoint<M, P, Q, 8, GP, JM, 256>);
+  if (grep_log2 == 0) return reinterpret_cast<const void*>(&mz::k_adjoint<M, P, Q, 1, GP, JM, 256>);
+  return nullptr;
 }
 
 template <bool M, bool P, bool Q>
-const void* adjoint_fn_t(const mz::PotDesc& pot, int grep_log2) {
-  if (!M) return reinterpret_cast<const void*>(&mz::k_adjoint<M, P, Q, 1, false, 0>);
-  if (pot.gpair) return adjoint_fn_g<M, P, Q, true, 0>(grep_log2);
-  if (pot.njm == 1) return adjoint_fn_g<M, P, Q, false, 1>(grep_log2);
-  if (pot.njm == 2) return adjoint_fn_g<M, P, Q, false, 2>(grep_log2);
-  return adjoint_fn_g<M, P, Q, false, 0>(grep_log2);
+const void* adjoint_fn_t(const mz::PotDesc& pot, int grep_log2, int blk) {
+  if (!M) {
+    if (grep_log2 != 0) return nullptr;
+    return blk == 512 ? reinterpret_cast<const void*>(&mz::k_adjoint<M, P, Q, 1, false, 0, 512>)
+                      : reinterpret_cast<const void*>(&mz::k_adjoint<M, P, Q, 1, false, 0, 256>);
+  }
+  if (pot.gpair) return adjoint_fn_g<M, P, Q, true, 0>(grep_log2, blk);
+  if (pot.njm == 1) return adjoint_fn_g<M, P, Q, false, 1>(grep_log2, blk);
+  if (pot.njm == 2) return adjoint_fn_g<M, P, Q, false, 2>(grep_log2, blk);
+  return adjoint_fn_g<M, P, Q, false, 0>(grep_log2, blk);
 }
 
-const void* adjoint_fn(const mz_ctx* c, int grep_log2) {
+const void* adjoint_fn(const mz_ctx* c, int g, int b) {
   const bool q = c->pot.quintic != 0;
   if (c->meam) {
-    if (c->point) return q ? adjoint_fn_t<true, true, true>(c->pot, grep_log2) : adjoint_fn_t<true, true, false>(c->pot, grep_log2);
-    return q ? adjoint_fn_t<true, false, true>(c->pot, grep_log2) : adjoint_fn_t<true, false, false>(c->pot, grep_log2);
+    if (c->point) return q ? adjoint_fn_t<true, true, true>(c->pot, g, b) : adjoint_fn_t<true, true, false>(c->pot, g, b);
+    return q ? adjoint_fn_t<true, false, true>(c->pot, g, b) : adjoint_fn_t<true, false, false>(c->pot, g, b);
   }
-  if (c->point) return q ? adjoint_fn_t<false, true, true>(c->pot, grep_log2) : adjoint_fn_t<false, true, false>(c->pot, grep_log2);
-  return q ? adjoint_fn_t<false, false, true>(c->pot, grep_log2) : adjoint_fn_t<false, false, false>(c->pot, grep_log2);
+  if (c->point) return q ? adjoint_fn_t<false, true, true>(c->pot, g, b) : adjoint_fn_t<false, true, false>(c->pot, g, b);
+  return q ? adjoint_fn_t<false, false, true>(c->pot, g, b) : adjoint_fn_t<false, false, false>(c->pot, g, b);
 }
 
 // Adjoint workgroups per candidate.  The launch holds eight times the
@@ -1118,7 +1164,7 @@ static int run_grad(mz_ctx* c, int ncand) {
   a.nchunks = X.nchunks;
   const int nslot = adjoint_nslot(X, ncand);
   void* kargs[] = {&a};
-  ADJ_HIP(hipLaunchKernel(X.fn, dim3(nslot, ncand), dim3(mz::kAdjBlock), kargs, X.dyn_bytes, st));
+  ADJ_HIP(hipLaunchKernel(X.fn, dim3(nslot, ncand), dim3(X.blk), kargs, X.dyn_bytes, st));
   if (c->timing) ADJ_HIP(hipEventRecord(c->ev[6], st));
   hipLaunchKernelGGL(mz::k_slotsum, dim3(X.ntab, ncand), dim3(256), 0, st, X.d_slots.p, X.ntab, nslot, X.d_tabsum.p);
   ADJ_HIP(hipGetLastError());

@@ -77,11 +77,11 @@ struct EvalBufs {
 struct AdjointExtra {
   bool ready = false;
   int cap = 0;
-  int nchunks = 0, ntab = 0, gco0 = 0, ngco = 0, dyn_bytes = 0, grep_log2 = 0, nrad = 0, rrep_log2 = -1,
+  int nchunks = 0, ntab = 0, gco0 = 0, ngco = 0, dyn_bytes = 0, grep_log2 = 0, nrad = 0, rrep_log2 = -1, blk = 256,
       rrep_stride = 0;
   const void* fn = nullptr;  // the k_adjoint instantiation of the potential
   int resident = 0;          // its workgroups resident on the whole GPU
-  DevBuf<Chunk> d_chunks;                 // chunks of <= 256 pairs
+  DevBuf<Chunk> d_chunks;                 // chunks of <= blk pairs and blk / 4 atoms
   DevBuf<int> d_owner;                    // owning atom of every pair
   DevBuf<double> d_J;                     // refresh Jacobians
   DevBuf<int> d_joff;
