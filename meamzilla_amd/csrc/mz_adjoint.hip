@@ -83,8 +83,9 @@ struct ChainArgs {
   double* grad;            // [cand][nknot]
 };
 
-// Partner data of every pair, structure of arrays (consecutive partners of
-// consecutive lanes read consecutive doubles): unit vector d (3), f, a f',
+// Partner data of every pair, planes of field pairs (consecutive partners of
+// consecutive lanes read consecutive 16-byte entries, one 16-byte LDS read
+// per plane): unit vector d (3), f, a f',
 // V = (w - a d) / r (3), the part of the dual normal to the pair over its
 // length, with which B_kj = (w_k . d_j - a_k cos) / r_k = V_k . d_j.
 constexpr int kPartnerFields = 8;
@@ -186,8 +187,8 @@ __device__ __forceinline__ GSpl gspl_of(const SplHead& H, int gco0) {
 // extrapolation (extrap; fl 0 left, 1 right; t the offset); keep is false for
 // a knot-style cosine at or beyond x_n (1 + 1e-8), whose term is absent.
 template <bool POINT, bool Q, int NC>
-__device__ __forceinline__ void ang_at(const GSpl& G, const double* gt, double cs, double& v, double& dvt, double& t,
-                                       int& fl, bool& extrap, bool& keep) {
+__device__ __forceinline__ void ang_at(const GSpl& G, const double2* gt2, int ps, double cs, double& v, double& dvt,
+                                       double& t, int& fl, bool& extrap, bool& keep) {
   extrap = false;
   keep = true;
   if (POINT && (cs < G.x0 || cs > G.xn)) {
@@ -205,10 +206,10 @@ __device__ __forceinline__ void ang_at(const GSpl& G, const double* gt, double c
   const double flo = fmin(fmax(floor(u), 0.0), G.kmax);
   t = u - flo;
   fl = (int)flo;
-  const double2* C = reinterpret_cast<const double2*>(gt + G.rec * NC) + fl * (NC / 2);
-  const double2 c01 = C[0], c23 = C[1];
+  const double2* C = gt2 + G.rec + fl;
+  const double2 c01 = C[0], c23 = C[ps];
   if (Q) {
-    const double2 c45 = C[2];
+    const double2 c45 = C[2 * ps];
     v = c01.x + t * (c01.y + t * (c23.x + t * (c23.y + t * (c45.x + t * c45.y))));
     dvt = c01.y + t * (2.0 * c23.x + t * (3.0 * c23.y + t * (4.0 * c45.x + t * 5.0 * c45.y)));
   } else {
@@ -217,6 +218,17 @@ __device__ __forceinline__ void ang_at(const GSpl& G, const double* gt, double c
   }
   if (!POINT) keep = cs < G.xcut;
 }
+
+// One visit of the triplet sweep: partner pair k and its record, the cosine,
+// the g spline (gi), its interval fl and shift t (or the extrapolation side
+// and offset), whether the term is kept, and the interval's coefficients.
+struct AdjVis {
+  int k, gi, fl;
+  bool extrap, keep;
+  double4 Pa, Pb;
+  double cs, t;
+  double2 c01, c23, c45;
+};
 
 // GR: lane replicas of the g-coefficient adjoints (32 keeps the 32 lanes that
 // one LDS cycle serves on distinct replicas; fewer when LDS is short).
@@ -230,7 +242,7 @@ __global__ void __launch_bounds__(BLK) k_adjoint(const AdjArgs A) {
   extern __shared__ __attribute__((aligned(64))) char dyn[];
   constexpr int NJ = JM > 0 ? JM : 1;
   constexpr int NA = BLK / 4;  // atoms per chunk: phase B takes four lanes per atom
-  __shared__ double pd[kPartnerFields + 2 * JM][BLK];
+  __shared__ double2 pd[(kPartnerFields + 2 * JM) / 2][BLK];  // partner data, planes of field pairs
   __shared__ double push[2 + JM][BLK];
   __shared__ double psi[NA], Up[NA];
   __shared__ int poff[NA + 1];
@@ -251,14 +263,19 @@ __global__ void __launch_bounds__(BLK) k_adjoint(const AdjArgs A) {
   const int ghb = A.pot.nspl * (int)sizeof(SplHead);              // all spline heads, staged
   const SplHead* HD = reinterpret_cast<const SplHead*>(dyn);
   const SplHead* GH = HD + A.pot.fn_off[4];
-  double* gt = reinterpret_cast<double*>(dyn + ghb);              // g coefficients, [record][NC]
-  double* atab = gt + (MEAM ? A.ngco * NC : 0);                   // coefficient + head adjoints
+  // g coefficients as planes of pairs: plane m holds (c_2m, c_2m+1) of every record
+  double2* gt2 = reinterpret_cast<double2*>(dyn + ghb);
+  double* atab = reinterpret_cast<double*>(gt2) + (MEAM ? A.ngco * NC : 0);  // coefficient + head adjoints
   double* gacc = atab + A.ntab;                                   // [g record][coef][replica]
   for (int q = tid; q < ghb / 16; q += BLK) reinterpret_cast<int4*>(dyn)[q] = reinterpret_cast<const int4*>(tbl)[q];
   if (MEAM) {
     const SplInterval* src = reinterpret_cast<const SplInterval*>(A.tables + (size_t)cand * A.table_bytes +
                                                                   sizeof(SplHead) * A.pot.nspl);
-    for (int q = tid; q < A.ngco * NC; q += BLK) gt[q] = src[A.gco0 + q / NC].c[q % NC];
+    for (int q = tid; q < A.ngco * (NC / 2); q += BLK) {
+      const int m = q / A.ngco;
+      const SplInterval& I = src[A.gco0 + q - m * A.ngco];
+      gt2[q] = make_double2(I.c[2 * m], I.c[2 * m + 1]);
+    }
   }
   const int nacc = A.ntab + (MEAM ? A.ngco * NC * R : 0);
   for (int q = tid; q < nacc; q += BLK) atab[q] = 0.0;
@@ -414,12 +431,11 @@ __global__ void __launch_bounds__(BLK) k_adjoint(const AdjArgs A) {
       const double pv[kPartnerFields] = {ux,      uy, uz, fj, aj * dfj, invr * (wx - aj * ux), invr * (wy - aj * uy),
                                          invr * (wz - aj * uz)};
 #pragma unroll
-      for (int q = 0; q < kPartnerFields; ++q) pd[q][p] = pv[q];
+      for (int q = 0; q < kPartnerFields / 2; ++q) pd[q][p] = make_double2(pv[2 * q], pv[2 * q + 1]);
       push[0][p] = 0.0;
 #pragma unroll
       for (int m = 0; m < JM; ++m) {
-        pd[kPartnerFields + 2 * m][p] = pvj[m];
-        pd[kPartnerFields + 2 * m + 1][p] = aj * dpj[m];
+        pd[kPartnerFields / 2 + m][p] = make_double2(pvj[m], aj * dpj[m]);
         push[2 + m][p] = 0.0;
       }
     }
@@ -492,48 +508,76 @@ __global__ void __launch_bounds__(BLK) k_adjoint(const AdjArgs A) {
     GSpl Gq[NJ];  // q (q2) of the atom's type
 #pragma unroll
     for (int m = 0; m < JM; ++m) Gq[m] = gspl_of(HD[A.pot.fn_off[6 + 2 * m] + ti], A.gco0);
-    int kl = l;
+    // Each visit locates its cosine and reads its g coefficients at the top;
+    // the partner record of the next visit is read at the end, behind the
+    // atomics (reads stay inside the atom's pairs).  Prefetching a whole
+    // visit ahead needs registers that cost a wave per SIMD, and measured
+    // slower.
+    auto fetch_partner = [&](int kq, AdjVis& v) {
+      v.k = kq;
+      const double2 p0 = pd[0][kq], p1 = pd[1][kq], p2 = pd[2][kq], p3 = pd[3][kq];
+      v.Pa = make_double4(p0.x, p0.y, p1.x, p1.y);
+      v.Pb = make_double4(p2.x, p2.y, p3.x, p3.y);
+      v.cs = ux * v.Pa.x + uy * v.Pa.y + uz * v.Pa.z;
+      v.gi = GP ? gfix + pair_ij(tj, ptyp[kq], nt) : gfix;
+    };
+    auto locate_load = [&](AdjVis& v) {
+      const GSpl G = GP ? gspl_of(GH[v.gi], A.gco0) : G0;
+      v.extrap = false;
+      v.keep = true;
+      if (POINT && (v.cs < G.x0 || v.cs > G.xn)) {
+        v.extrap = true;
+        const bool left = v.cs < G.x0;
+        v.t = v.cs - (left ? G.x0 : G.xn);
+        v.fl = left ? 0 : 1;
+      } else {
+        // interval index clamped as an integer (the cosine lies in [-1, 1])
+        const double u = (v.cs - G.x0) * G.inv;
+        const int kmax = GP ? (int)G.kmax : kmax0;
+        int fl = (int)floor(u);
+        fl = fl < 0 ? 0 : (fl > kmax ? kmax : fl);
+        v.fl = fl;
+        v.t = u - (double)fl;
+        if (!POINT) v.keep = v.cs < G.xcut;
+      }
+      const double2* C = gt2 + G.rec + v.fl;
+      v.c01 = C[0];
+      v.c23 = C[A.ngco];
+      if (Q) v.c45 = C[2 * A.ngco];
+    };
+    int kl = (l + 1 == n) ? 0 : l + 1;
+    AdjVis cur;
+    fetch_partner(kb + kl, cur);
     for (int m = 0; m < trips; ++m) {
       kl = (kl + 1 == n) ? 0 : kl + 1;
-      const int k = kb + kl;
-      const double4 Pa = make_double4(pd[0][k], pd[1][k], pd[2][k], pd[3][k]);
-      const double4 Pb = make_double4(pd[4][k], pd[5][k], pd[6][k], pd[7][k]);
-      const double cs = ux * Pa.x + uy * Pa.y + uz * Pa.z;
-      const int gi = GP ? gfix + pair_ij(tj, ptyp[k], nt) : gfix;
+      locate_load(cur);
+      const int k = cur.k;
+      const double4 Pa = cur.Pa, Pb = cur.Pb;
+      const double cs = cur.cs;
+      const int gi = cur.gi;
       const GSpl G = GP ? gspl_of(GH[gi], A.gco0) : G0;
       const double ginv = G.inv;
       const double upi = GP ? up * ginv : upi0;
-      const double2* gtg = reinterpret_cast<const double2*>(gt + G.rec * NC);
-      double gv, dgt, t;
-      int fl;
-      bool extrap = false, keep = true;
-      if (POINT && (cs < G.x0 || cs > G.xn)) {
-        extrap = true;
-        const bool left = cs < G.x0;
-        const double e = cs - (left ? G.x0 : G.xn);
+      const double t = cur.t;
+      const int fl = cur.fl;
+      const bool extrap = POINT && cur.extrap, keep = cur.keep;
+      double gv, dgt;
+      if (POINT && extrap) {
+        const bool left = fl == 0;
+        const double e = t;
         const double g1 = left ? G.gL : G.gR, g2 = left ? G.g2L : G.g2R;
         gv = (left ? G.y0 : G.yn) + g1 * e + 0.5 * g2 * e * e;
         dgt = (g1 + g2 * e) / ginv;
-        t = e;
-        fl = left ? 0 : 1;
       } else {
-        // interval index clamped as an integer (the cosine lies in [-1, 1])
-        const double u = (cs - G.x0) * ginv;
-        const int kmax = GP ? (int)G.kmax : kmax0;
-        fl = (int)floor(u);
-        fl = fl < 0 ? 0 : (fl > kmax ? kmax : fl);
-        t = u - (double)fl;
-        const double2* C = gtg + fl * (NC / 2);
-        const double2 c01 = C[0], c23 = C[1];
+        const double2 c01 = cur.c01, c23 = cur.c23;
         if (Q) {
-          const double2 c45 = C[2];
+          const double2 c45 = cur.c45;
           gv = c01.x + t * (c01.y + t * (c23.x + t * (c23.y + t * (c45.x + t * c45.y))));
           dgt = c01.y + t * (2.0 * c23.x + t * (3.0 * c23.y + t * (4.0 * c45.x + t * 5.0 * c45.y)));
         } else {
           gv = c01.x + t * (c01.y + t * (c23.x + t * c23.y));
           dgt = c01.y + t * (2.0 * c23.x + t * 3.0 * c23.y);
         }
-        if (!POINT) keep = cs < G.xcut;
       }
       if (!keep) {
         gv = 0.0;
@@ -586,9 +630,10 @@ __global__ void __launch_bounds__(BLK) k_adjoint(const AdjArgs A) {
         double qv, dqt, tq;
         int flq;
         bool exq, kq;
-        ang_at<POINT, Q, NC>(Gq[m], gt, cs, qv, dqt, tq, flq, exq, kq);
+        ang_at<POINT, Q, NC>(Gq[m], gt2, A.ngco, cs, qv, dqt, tq, flq, exq, kq);
         if (!kq) continue;
-        const double pk = pd[kPartnerFields + 2 * m][k], Apk = pd[kPartnerFields + 2 * m + 1][k];
+        const double2 pq = pd[kPartnerFields / 2 + m][k];
+        const double pk = pq.x, Apk = pq.y;
         const double pj = pvj[m], Apj = aj * dpj[m];
         const double Xq = lamE * qv + dqt * Gq[m].inv * Bs;
         Rq[m] += pk * Xq + qv * Apk;
@@ -616,6 +661,7 @@ __global__ void __launch_bounds__(BLK) k_adjoint(const AdjArgs A) {
           }
         }
       }
+      fetch_partner(kb + kl, cur);  // the next visit's partner, read behind this visit's atomics
     }
     Phi = Rj;
   }
@@ -626,7 +672,7 @@ __global__ void __launch_bounds__(BLK) k_adjoint(const AdjArgs A) {
   __syncthreads();
   const int RR = A.rrep_log2 >= 0 ? 1 << A.rrep_log2 : 0;
   const int RS = A.rrep_stride;
-  double* rrep = &pd[0][0];
+  double* rrep = &pd[0][0].x;
   for (int q = tid; q < RS * RR; q += BLK) rrep[q] = 0.0;
   if (MEAM && has) {
     Phi += push[0][p];

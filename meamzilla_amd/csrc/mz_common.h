@@ -81,9 +81,12 @@ struct PotDesc {
   int32_t njm;          // jmeam styles: extra p_ij p_ik q(cos) energy terms (1, or 2 for jmeam2)
 };
 
-// Doubles per g-interval record in the atom kernel's compact LDS copy
-// (c0..c3 for cubic, c0..c5 padded to 8 for quintic splines).
-constexpr int g_record_doubles(bool quintic) { return quintic ? 8 : 4; }
+// Doubles per g-interval record in the compact LDS copies of the g tables
+// (c0..c3 for cubic, c0..c5 for quintic splines).  The copies are stored as
+// planes of coefficient pairs: plane m holds (c_2m, c_2m+1) of every record,
+// 16 bytes apart, so that lanes reading different intervals with one 16-byte
+// load touch different LDS banks (records 8 apart share a bank quad).
+constexpr int g_record_doubles(bool quintic) { return quintic ? 6 : 4; }
 
 // Packs an image triple into one 32-bit word (8 bits each, signed).
 inline uint32_t pack_image(int ix, int iy, int iz) {

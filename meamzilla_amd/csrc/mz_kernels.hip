@@ -264,7 +264,7 @@ __device__ __forceinline__ void eval_located(const SplHead& H, const SplInterval
 // Shared part of a g evaluation: interval and shift of cos (knot positions are
 // common to all candidates), or the extrapolation side (point style).
 struct GLoc {
-  int off;    // record offset (doubles) of the interval in a compact g table
+  int off;    // interval (record) index in the spline's compact g records
   int side;   // 0 inside, 1 left of x_0, 2 right of x_n (point style)
   double t;   // shift in the interval, or cos - end when extrapolating
 };
@@ -281,14 +281,15 @@ __device__ __forceinline__ GLoc g_locate(const SplHead& Hg, double cs) {
   const double u = (cs - Hg.x0) * Hg.invstep;
   const double fl = fmin(fmax(floor(u), 0.0), (double)(Hg.n - 2));
   L.t = u - fl;
-  L.off = g_record_doubles(Q) * (int)fl;
+  L.off = (int)fl;
   L.side = 0;
   return L;
 }
 
 // g and dg/dcos of one candidate at a located cosine.
 template <bool POINT, bool Q>
-__device__ __forceinline__ void g_value(const SplHead& Hg, const double* gtg, const GLoc& L, double& gv, double& gp) {
+__device__ __forceinline__ void g_value(const SplHead& Hg, const double2* gtg, int ps, const GLoc& L, double& gv,
+                                        double& gp) {
   if (POINT && L.side != 0) {
     const bool left = L.side == 1;
     const double e = L.t;
@@ -298,11 +299,11 @@ __device__ __forceinline__ void g_value(const SplHead& Hg, const double* gtg, co
     return;
   }
   const double t = L.t;
-  const double2* C = reinterpret_cast<const double2*>(gtg + L.off);
-  const double2 c01 = C[0], c23 = C[1];
+  const double2* C = gtg + L.off;
+  const double2 c01 = C[0], c23 = C[ps];
   double dgt;
   if (Q) {
-    const double2 c45 = C[2];
+    const double2 c45 = C[2 * ps];
     gv = c01.x + t * (c01.y + t * (c23.x + t * (c23.y + t * (c45.x + t * c45.y))));
     dgt = c01.y + t * (2.0 * c23.x + t * (3.0 * c23.y + t * (4.0 * c45.x + t * 5.0 * c45.y)));
   } else {
@@ -375,9 +376,13 @@ __global__ void __launch_bounds__(BLOCK) k_atoms(const AtomArgs args) {
     }
     if (MEAM) {
       const SplInterval* gsrc = reinterpret_cast<const SplInterval*>(tb + sizeof(SplHead) * args.pot.nspl);
-      double* gdst = reinterpret_cast<double*>(cb + tbytes + ghb);
-      for (int q = tid; q < args.pot.ngco * NG; q += BLOCK)
-        gdst[q] = (q % NG) < 6 ? gsrc[args.pot.gco0 + q / NG].c[q % NG] : 0.0;
+      double2* gdst = reinterpret_cast<double2*>(cb + tbytes + ghb);
+      const int ng = args.pot.ngco;
+      for (int q = tid; q < ng * (NG / 2); q += BLOCK) {
+        const int m = q / ng;
+        const SplInterval& I = gsrc[args.pot.gco0 + q - m * ng];
+        gdst[q] = make_double2(I.c[2 * m], I.c[2 * m + 1]);
+      }
     }
   }
   auto HDc = [&](int c) { return reinterpret_cast<const SplHead*>(dyn + (size_t)c * cbytes); };
@@ -386,7 +391,8 @@ __global__ void __launch_bounds__(BLOCK) k_atoms(const AtomArgs args) {
                                                 sizeof(SplHead) * args.pot.nspl);
   };
   auto GHc = [&](int c) { return HDc(c) + args.pot.fn_off[4]; };  // g heads by element type (type, pair of types with GP)
-  auto GTc = [&](int c) { return reinterpret_cast<const double*>(dyn + (size_t)c * cbytes + tbytes + ghb); };
+  auto GTc = [&](int c) { return reinterpret_cast<const double2*>(dyn + (size_t)c * cbytes + tbytes + ghb); };
+  const int ngc = args.pot.ngco;  // plane stride of the compact g records
   for (int a = tid; a <= ch.na; a += BLOCK) sm.poff[a] = args.atom_poff[ch.a0 + a] - ch.p0;
   __syncthreads();
 
@@ -516,7 +522,7 @@ __global__ void __launch_bounds__(BLOCK) k_atoms(const AtomArgs args) {
       const int ti = args.types[ch.a0 + ia[s]];
       const int gfix = GP ? ti * (nt * (nt + 1) / 2) : ti;  // g of the lane's atom (and, with GP, first pair)
       const SplHead& Hg0 = GHc(0)[gfix];
-      const int goff0 = (Hg0.coef_off - args.pot.gco0) * NG;
+      const int goff0 = Hg0.coef_off - args.pot.gco0;
       const double xj = ux[s], yj = uy[s], zj = uz[s];
       double S[CT][5], T[NJ][CT][5];
 #pragma unroll
@@ -532,13 +538,13 @@ __global__ void __launch_bounds__(BLOCK) k_atoms(const AtomArgs args) {
         const double cs = xj * P.x + yj * P.y + zj * P.z;
         const int gi = GP ? gfix + pair_ij(tjs[s], sm.ptype[k], nt) : gfix;
         const SplHead& Hg = GP ? GHc(0)[gi] : Hg0;
-        const int goff = GP ? (Hg.coef_off - args.pot.gco0) * NG : goff0;
+        const int goff = GP ? Hg.coef_off - args.pot.gco0 : goff0;
         const GLoc L = g_locate<POINT, Q>(Hg, cs);
         const bool zero = k == p || (cut && !(cs < Hg.xcut));
 #pragma unroll
         for (int c = 0; c < CT; ++c) {
           double gv, gp;
-          g_value<POINT, Q>(GHc(c)[gi], GTc(c) + goff, L, gv, gp);
+          g_value<POINT, Q>(GHc(c)[gi], GTc(c) + goff, ngc, L, gv, gp);
           const double fk = zero ? 0.0 : (c == 0 ? P.w : sm.fx[c > 0 ? c - 1 : 0][k]);
           const double b = gp * fk;
           S[c][0] += gv * fk;
@@ -555,11 +561,11 @@ __global__ void __launch_bounds__(BLOCK) k_atoms(const AtomArgs args) {
           const SplHead& Hq = HDc(0)[iq];
           const GLoc Lq = g_locate<POINT, Q>(Hq, cs);
           const bool zq = k == p || (!POINT && !(cs < Hq.xcut));
-          const int qoff = (Hq.coef_off - args.pot.gco0) * NG;
+          const int qoff = Hq.coef_off - args.pot.gco0;
 #pragma unroll
           for (int c = 0; c < CT; ++c) {
             double qv, qp;
-            g_value<POINT, Q>(HDc(c)[iq], GTc(c) + qoff, Lq, qv, qp);
+            g_value<POINT, Q>(HDc(c)[iq], GTc(c) + qoff, ngc, Lq, qv, qp);
             const double pk = zq ? 0.0 : sm.px[m * CT + c][k];
             const double b = qp * pk;
             T[m][c][0] += qv * pk;
