@@ -143,18 +143,6 @@ __device__ __forceinline__ void acc_point(double* tab, const CoefSink& sk, int n
   acc_interval<Q>(sk, H.coef_off + k, t, al, be * H.invstep);
 }
 
-// Knot-style location (same rule as eval_knot); false beyond x_n (1 + 1e-8).
-__device__ __forceinline__ bool locate_knot(const SplHead& H, const SplInterval* IVs, double r, int& k, double& t) {
-  if (!(r < H.xcut)) return false;
-  const int nm2 = H.n - 2;
-  k = (int)floor((r - H.x0) * H.invstep);
-  k = k < 0 ? 0 : (k > nm2 ? nm2 : k);
-  while (k > 0 && IVs[k].xk > r) --k;
-  while (k < nm2 && IVs[k + 1].xk <= r) ++k;
-  t = (r - IVs[k].xk) * IVs[k].invh;
-  return true;
-}
-
 // Adjoint of one radial function of a pair (value weight al, d/dr weight be).
 template <bool POINT, bool Q>
 __device__ __forceinline__ void acc_radial(double* tab, const CoefSink& sk, int ncoef, int s, const SplHead& H,
@@ -265,7 +253,9 @@ __global__ void __launch_bounds__(BLK) k_adjoint(const AdjArgs A) {
   const SplHead* GH = HD + A.pot.fn_off[4];
   // g coefficients as planes of pairs: plane m holds (c_2m, c_2m+1) of every record
   double2* gt2 = reinterpret_cast<double2*>(dyn + ghb);
-  double* atab = reinterpret_cast<double*>(gt2) + (MEAM ? A.ngco * NC : 0);  // coefficient + head adjoints
+  // knot positions of every interval record, for locating r (LDS reads only)
+  KnotRec* kg = reinterpret_cast<KnotRec*>(gt2 + (MEAM ? A.ngco * (NC / 2) : 0));
+  double* atab = reinterpret_cast<double*>(kg + ncoef);  // coefficient + head adjoints
   double* gacc = atab + A.ntab;                                   // [g record][coef][replica]
   for (int q = tid; q < ghb / 16; q += BLK) reinterpret_cast<int4*>(dyn)[q] = reinterpret_cast<const int4*>(tbl)[q];
   if (MEAM) {
@@ -277,6 +267,7 @@ __global__ void __launch_bounds__(BLK) k_adjoint(const AdjArgs A) {
       gt2[q] = make_double2(I.c[2 * m], I.c[2 * m + 1]);
     }
   }
+  for (int q = tid; q < ncoef; q += BLK) kg[q] = KnotRec{IV[q].xk, IV[q].invh};
   const int nacc = A.ntab + (MEAM ? A.ngco * NC * R : 0);
   for (int q = tid; q < nacc; q += BLK) atab[q] = 0.0;
 
@@ -353,8 +344,8 @@ __global__ void __launch_bounds__(BLK) k_adjoint(const AdjArgs A) {
   const int ia = ig - ch.a0;
   int pij = 0;
   double r = 0, invr = 0, ux = 0, uy = 0, uz = 0, fj = 0, dfj = 0, drho = 0, aj = 0, wx = 0, wy = 0, wz = 0;
-  int rrec = -1, frec = -1;  // located rho and f interval records (knot style)
-  double trho = 0, tf = 0, rinv = 0, finv = 0;
+  int rrec = -1, frec = -1, phrec = -1;  // located rho, f and phi interval records (knot style)
+  double trho = 0, tf = 0, tphi = 0, rinv = 0, finv = 0, phinv = 0;
   // jmeam: p and p' of the pair per term, and the located p interval (knot style)
   double pvj[NJ], dpj[NJ], ptt[NJ], pinv[NJ];
   int prec[NJ];
@@ -371,8 +362,17 @@ __global__ void __launch_bounds__(BLK) k_adjoint(const AdjArgs A) {
     uz = d[2] / r;
     pij = pair_ij(ti, tj, nt);
     double v, dv;
+    if (!POINT) {
+      // phi is located here too, beside rho and f, for its adjoint in phase D
+      const SplHead& Hp = HD[A.pot.fn_off[0] + pij];
+      int k;
+      if (locate_knot(Hp, kg + Hp.coef_off, r, k, tphi)) {
+        phrec = Hp.coef_off + k;
+        phinv = Hp.invstep;
+      }
+    }
     if (A.pot.nodens) {
-      // pair styles: phi only, nothing to locate here
+      // pair styles: phi only
     } else if (POINT) {
       if (eval_radial<POINT, Q>(HD[A.pot.fn_off[1] + tj], IV, r, v, dv)) drho = dv;
       if (MEAM && eval_radial<POINT, Q>(HD[A.pot.fn_off[3] + pij], IV, r, v, dv)) {
@@ -384,7 +384,7 @@ __global__ void __launch_bounds__(BLK) k_adjoint(const AdjArgs A) {
       // adjoints of phase D
       const SplHead& Hr = HD[A.pot.fn_off[1] + tj];
       int k;
-      if (locate_knot(Hr, IV + Hr.coef_off, r, k, trho)) {
+      if (locate_knot(Hr, kg + Hr.coef_off, r, k, trho)) {
         rrec = Hr.coef_off + k;
         rinv = Hr.invstep;
         poly<Q>(IV[rrec], trho, rinv, v, dv);
@@ -392,7 +392,7 @@ __global__ void __launch_bounds__(BLK) k_adjoint(const AdjArgs A) {
       }
       if (MEAM) {
         const SplHead& Hf = HD[A.pot.fn_off[3] + pij];
-        if (locate_knot(Hf, IV + Hf.coef_off, r, k, tf)) {
+        if (locate_knot(Hf, kg + Hf.coef_off, r, k, tf)) {
           frec = Hf.coef_off + k;
           finv = Hf.invstep;
           poly<Q>(IV[frec], tf, finv, v, dv);
@@ -411,7 +411,7 @@ __global__ void __launch_bounds__(BLK) k_adjoint(const AdjArgs A) {
         }
       } else {
         int k;
-        if (locate_knot(Hp, IV + Hp.coef_off, r, k, ptt[m])) {
+        if (locate_knot(Hp, kg + Hp.coef_off, r, k, ptt[m])) {
           prec[m] = Hp.coef_off + k;
           pinv[m] = Hp.invstep;
           poly<Q>(IV[prec[m]], ptt[m], pinv[m], v, dv);
@@ -685,14 +685,15 @@ __global__ void __launch_bounds__(BLK) k_adjoint(const AdjArgs A) {
     const CoefSink sk = RR ? CoefSink{rrep + (tid & (RR - 1)) * RS, NC, 1} : CoefSink{atab, 6, 1};
     const double ps = psi[ia], up = Up[ia];
     const int sphi = A.pot.fn_off[0] + pij, srho = A.pot.fn_off[1] + tj;
-    acc_radial<POINT, Q>(atab, sk, ncoef, sphi, HD[sphi], IV, r, 0.5 * lamE, 0.5 * aj);
     if (POINT) {
+      acc_radial<POINT, Q>(atab, sk, ncoef, sphi, HD[sphi], IV, r, 0.5 * lamE, 0.5 * aj);
       if (!A.pot.nodens) acc_radial<POINT, Q>(atab, sk, ncoef, srho, HD[srho], IV, r, ps, aj * up);
       if (MEAM) {
         const int sf = A.pot.fn_off[3] + pij;
         acc_radial<POINT, Q>(atab, sk, ncoef, sf, HD[sf], IV, r, Phi, Phip);
       }
     } else {
+      if (phrec >= 0) acc_interval<Q>(sk, phrec, tphi, 0.5 * lamE, 0.5 * aj * phinv);
       if (rrec >= 0) acc_interval<Q>(sk, rrec, trho, ps, aj * up * rinv);
       if (MEAM && frec >= 0) acc_interval<Q>(sk, frec, tf, Phi, Phip * finv);
     }
@@ -998,7 +999,8 @@ int setup_adjoint(mz_ctx* c) {
   X.gco0 = c->meam ? c->coef_off[c->pot.fn_off[4]] : 0;
   X.ngco = c->meam ? c->ncoef - X.gco0 : 0;
   const int nc_g = c->pot.quintic ? 6 : 4;
-  const int fixed = (X.ngco * nc_g + X.ntab) * (int)sizeof(double) + c->nspl * (int)sizeof(mz::SplHead);
+  const int fixed = (X.ngco * nc_g + X.ntab) * (int)sizeof(double) + c->nspl * (int)sizeof(mz::SplHead) +
+                    c->ncoef * (int)sizeof(mz::KnotRec);
   int maxlds = 0, cus = 0;
   if (hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device) != hipSuccess ||
       hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess || cus < 1)

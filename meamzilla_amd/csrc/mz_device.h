@@ -46,19 +46,37 @@ __device__ __forceinline__ void poly(const SplInterval& I, double t, double invs
   }
 }
 
-// Knot-style evaluation (Spline::get_knotshift src/spline.cpp:221-246 then
-// splint_comb by knot).  Returns false when r lies beyond x_n (1 + 1e-8).
-template <bool Q>
-__device__ __forceinline__ bool eval_knot(const SplHead& H, const SplInterval* IV, double r, double& v, double& d) {
+// Knot positions of one interval record, x_k and 1 / (x_(k+1) - x_k): what
+// locating r needs of a spline table.  The knots sit at the same x for every
+// candidate, so the kernels stage one copy of them in LDS.
+struct KnotRec {
+  double xk, invh;
+};
+
+// Knot-style location (Spline::get_knotshift src/spline.cpp:221-246): the
+// interval k with x_k <= r < x_(k+1) (clamped to the first and last) and the
+// shift t in it, from the spline's records (SplInterval or KnotRec).  Returns
+// false when r lies beyond x_n (1 + 1e-8).
+template <class Rec>
+__device__ __forceinline__ bool locate_knot(const SplHead& H, const Rec* I, double r, int& k, double& t) {
   if (!(r < H.xcut)) return false;
   const int nm2 = H.n - 2;
-  int k = (int)floor((r - H.x0) * H.invstep);
+  k = (int)floor((r - H.x0) * H.invstep);
   k = k < 0 ? 0 : (k > nm2 ? nm2 : k);
-  while (k > 0 && IV[k].xk > r) --k;
-  while (k < nm2 && IV[k + 1].xk <= r) ++k;
-  const SplInterval& I = IV[k];
-  const double t = (r - I.xk) * I.invh;
-  poly<Q>(I, t, H.invstep, v, d);
+  while (k > 0 && I[k].xk > r) --k;
+  while (k < nm2 && I[k + 1].xk <= r) ++k;
+  t = (r - I[k].xk) * I[k].invh;
+  return true;
+}
+
+// Knot-style evaluation (locate_knot, then splint_comb by knot).  Returns
+// false when r lies beyond x_n (1 + 1e-8).
+template <bool Q>
+__device__ __forceinline__ bool eval_knot(const SplHead& H, const SplInterval* IV, double r, double& v, double& d) {
+  int k;
+  double t;
+  if (!locate_knot(H, IV, r, k, t)) return false;
+  poly<Q>(IV[k], t, H.invstep, v, d);
   return true;
 }
 

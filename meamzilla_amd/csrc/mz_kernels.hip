@@ -238,15 +238,7 @@ __device__ __forceinline__ bool locate_radial(const SplHead& H, const SplInterva
     t = __dmul_rn(__dadd_rn(delta, -__dmul_rn((double)k, H.step)), H.invstep);
     return true;
   }
-  if (!(r < H.xcut)) return false;
-  const SplInterval* I = IV + H.coef_off;
-  const int nm2 = H.n - 2;
-  k = (int)floor((r - H.x0) * H.invstep);
-  k = k < 0 ? 0 : (k > nm2 ? nm2 : k);
-  while (k > 0 && I[k].xk > r) --k;
-  while (k < nm2 && I[k + 1].xk <= r) ++k;
-  t = (r - I[k].xk) * I[k].invh;
-  return true;
+  return locate_knot(H, IV + H.coef_off, r, k, t);
 }
 
 // Value and d/dr of one candidate's spline at a located r.
@@ -399,6 +391,8 @@ __global__ void __launch_bounds__(BLOCK) k_atoms(const AtomArgs args) {
   const int nt = args.pot.nt;
   const double* box = args.box + 9 * ch.cell;
   const SplHead* HD0 = HDc(0);
+  // knots sit at the same x for all candidates: candidate 0's records locate r
+  // (staging a shared copy of them in LDS per workgroup measured slower)
   const SplInterval* IV0 = IVc(0);
 
   // per-lane pair state
