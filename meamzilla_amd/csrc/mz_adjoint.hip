@@ -22,6 +22,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -226,7 +227,7 @@ struct AdjVis {
 // term, the sweep a q evaluation, a push and the q-coefficient adjoints
 // (the g term with psi -> lambda_E, U' -> 1, f -> p).
 template <bool MEAM, bool POINT, bool Q, int GR, bool GP, int JM, int BLK>
-__global__ void __launch_bounds__(BLK) k_adjoint(const AdjArgs A) {
+__global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu((MEAM && !GP && JM == 0) ? 4 : 1))) k_adjoint(const AdjArgs A) {
   extern __shared__ __attribute__((aligned(64))) char dyn[];
   constexpr int NJ = JM > 0 ? JM : 1;
   constexpr int NA = BLK / 4;  // atoms per chunk: phase B takes four lanes per atom
@@ -240,6 +241,11 @@ __global__ void __launch_bounds__(BLK) k_adjoint(const AdjArgs A) {
   const int tid = threadIdx.x;
   const int ncoef = A.pot.ncoef;
   constexpr int R = GR;
+  // the paired sweep (two own pairs per lane) for styles whose g depends on
+  // the centre atom only and that carry no p p q terms; its instantiations
+  // are held to 128 VGPRs (four waves per SIMD, the kernel attribute), a few
+  // values live across the sweep spill around it, none inside
+  constexpr bool kCouples = MEAM && !GP && JM == 0;
   const int nt = A.pot.nt;
 
   // The candidate's interval records stay in global memory (cache-resident;
@@ -485,18 +491,28 @@ __global__ void __launch_bounds__(BLK) k_adjoint(const AdjArgs A) {
   // f_j X + U' g A_j to that of pair k (A = a f'), and the g-coefficient
   // adjoint of weights
   // psi f_j f_k + U' (A_j f_k + A_k f_j) (value) and U' f_j f_k (B_jk + B_kj)
-  // (slope).  The lane of pair j visits partners j+1 .. j+(n-1)/2 (mod n),
-  // plus j+n/2 for the first n/2 lanes when n is even (the forward pass's
-  // round-robin); the partner sums go through LDS atomics on distinct k.
+  // (slope).  Styles whose g depends on the centre atom alone and that have
+  // no p p q terms take the paired sweep below (two own pairs per lane, one
+  // partner record per step for both).  The others: the lane of pair j visits
+  // partners j+1 .. j+(n-1)/2 (mod n), plus j+n/2 for the first n/2 lanes
+  // when n is even (the forward pass's round-robin); the partner sums go
+  // through LDS atomics on distinct k.
   double Phi = 0.0, Phip = 0.0;
+  // knot style, meam: phase D's slope weights of the phi, rho and f
+  // intervals, formed before the sweep (fewer registers live across it)
+  double wphi = 0.0, wrho = 0.0, wf = 0.0;
   double Rq[NJ];  // jmeam: p-value adjoint of the lane's pair per term, own visits
 #pragma unroll
   for (int m = 0; m < NJ; ++m) Rq[m] = 0.0;
   if (MEAM && has) {
     const double ps = psi[ia], up = Up[ia];
+    if (!POINT) {
+      wphi = 0.5 * aj * phinv;
+      wrho = aj * up * rinv;
+      wf = up * aj * S1 * finv;
+    }
     const int kb = poff[ia], ke = poff[ia + 1];
     const int n = ke - kb, l = p - kb;
-    const int trips = ((n - 1) >> 1) + (((n & 1) == 0 && l < (n >> 1)) ? 1 : 0);
     const int gfix = GP ? ti * (nt * (nt + 1) / 2) : ti;  // g of the atom's type (and first pair with GP)
     const GSpl G0 = gspl_of(GP ? GH[gfix] : HD[A.pot.fn_off[4] + ti], A.gco0);
     const double Aj = aj * dfj;
@@ -504,6 +520,159 @@ __global__ void __launch_bounds__(BLK) k_adjoint(const AdjArgs A) {
     // lane constants: U' A_j, U' f_j, and U' / step of the atom's g
     const double upAj = up * Aj, upfj = up * fj, upi0 = up * G0.inv;
     const int kmax0 = (int)G0.kmax;
+    if constexpr (kCouples) {
+    // One triplet {j, k} of the sweep in two steps: locate (the cosine of own
+    // pair j's direction u with partner k's, the g interval and shift, the
+    // interval's coefficients) and apply (g and g' at the cosine, pair j's
+    // f-value adjoint share added to Racc, the g-coefficient adjoints; returns
+    // pair k's share).  SPAN: the atom's g spline spans [-1, 1] (knot style),
+    // so no term is cut and the interval is min(trunc(u), n - 2); otherwise
+    // the interval is the clamped floor and, knot style, a term is absent for
+    // a cosine at or beyond x_n (1 + 1e-8) unless every g spline reaches
+    // beyond 1.
+    struct TriV {
+      double t;
+      int fl;
+      bool ex, keep;
+      double2 c01, c23, c45;
+    };
+    const bool gcut = !POINT && !A.pot.g_keep_all;
+    auto tri_locate = [&](double oux, double ouy, double ouz, const double4& Pa, TriV& v, auto span) {
+      const double cs = oux * Pa.x + ouy * Pa.y + ouz * Pa.z;
+      v.ex = false;
+      v.keep = true;
+      if (decltype(span)::value) {
+        const double u = (cs - G0.x0) * G0.inv;
+        const int k = (int)u;
+        v.fl = k < kmax0 ? k : kmax0;
+        v.t = u - (double)v.fl;
+        const double2* C = gt2 + G0.rec + v.fl;
+        v.c01 = C[0];
+        v.c23 = C[A.ngco];
+        if (Q) v.c45 = C[2 * A.ngco];
+        return;
+      }
+      if (POINT && (cs < G0.x0 || cs > G0.xn)) {
+        v.ex = true;
+        const bool left = cs < G0.x0;
+        v.t = cs - (left ? G0.x0 : G0.xn);
+        v.fl = left ? 0 : 1;
+        return;
+      }
+      const double u = (cs - G0.x0) * G0.inv;
+      int fl = (int)floor(u);
+      fl = fl < 0 ? 0 : (fl > kmax0 ? kmax0 : fl);
+      v.fl = fl;
+      v.t = u - (double)fl;
+      if (gcut) v.keep = cs < G0.xcut;
+      const double2* C = gt2 + G0.rec + fl;
+      v.c01 = C[0];
+      v.c23 = C[A.ngco];
+      if (Q) v.c45 = C[2 * A.ngco];
+    };
+    auto tri_apply = [&](double oux, double ouy, double ouz, double of, double oA, double oVx, double oVy,
+                         double oVz, const double4& Pa, const double4& Pb, const TriV& v, double& Racc,
+                         auto span) -> double {
+      const double t = v.t;
+      double gv, dgt;
+      if (POINT && v.ex) {
+        const bool left = v.fl == 0;
+        const double g1 = left ? G0.gL : G0.gR, g2 = left ? G0.g2L : G0.g2R;
+        gv = (left ? G0.y0 : G0.yn) + g1 * t + 0.5 * g2 * t * t;
+        dgt = (g1 + g2 * t) / G0.inv;
+      } else {
+        horner_d<Q>(v.c01, v.c23, v.c45, t, gv, dgt);
+      }
+      double fk = Pa.w, Ak = Pb.x;
+      if (!decltype(span)::value && !v.keep) {
+        gv = 0.0;
+        dgt = 0.0;
+        fk = 0.0;
+        Ak = 0.0;
+      }
+      const double Bs = fma(oVx, Pa.x, fma(oVy, Pa.y, fma(oVz, Pa.z, fma(Pb.y, oux, fma(Pb.z, ouy, Pb.w * ouz)))));
+      const double X = fma(ps, gv, upi0 * dgt * Bs);
+      const double ugv = up * gv;
+      const double ff = of * fk;
+      Racc = fma(fk, X, fma(ugv, Ak, Racc));
+      const double al = fma(ps, ff, up * fma(oA, fk, of * Ak));
+      if (POINT && v.ex) {
+        const double be = up * ff * Bs;
+        double* ghead = atab + ncoef * 6 + (A.pot.fn_off[4] + ti) * 6;
+        atomicAdd(ghead + v.fl, al);
+        atomicAdd(ghead + 2 + v.fl, al * t + be);
+        if (G0.quintic) atomicAdd(ghead + 4 + v.fl, 0.5 * al * t * t + be * t);
+      } else {
+        const double bt = upi0 * (ff * Bs);
+        double* cp = gacc + G0.rec * NC * R + (tid & (R - 1)) + v.fl * (NC * R);
+        const double m1 = fma(al, t, bt);
+        const double b2 = bt * t;
+        const double m2 = fma(t, m1, b2);
+        const double b3 = b2 * t;
+        const double m3 = fma(t, m2, b3);
+        atomicAdd(cp, al);
+        atomicAdd(cp + R, m1);
+        atomicAdd(cp + 2 * R, m2);
+        atomicAdd(cp + 3 * R, m3);
+        if (Q) {
+          const double b4 = b3 * t;
+          const double m4 = fma(t, m3, b4);
+          atomicAdd(cp + 4 * R, m4);
+          atomicAdd(cp + 5 * R, fma(t, m4, b4 * t));
+        }
+      }
+      return fma(of, X, ugv * oA);
+    };
+    // Couples: the atom's local pairs 2c and 2c + 1.  Both lanes of a couple
+    // hold both pairs and visit one partner pair per step for the two of them,
+    // so one partner-record read and one push serve two triplets.  The couples
+    // of an atom run the round-robin among themselves (couple c visits couples
+    // c + 1 .. c + (mc - 1) / 2, plus c + mc / 2 for the first mc / 2 when mc
+    // is even): the even lane takes the visited couple's first pair, the odd
+    // lane its second.  The even lane also takes the couple's own triplet
+    // {2c, 2c + 1} and, when n is odd, the odd lane visits the last pair,
+    // which is in no couple and visits nobody.
+    auto sweep = [&](auto span) {
+      const int mc = n >> 1;
+      double Rs = 0.0;
+      if (l < 2 * mc) {
+        const int cpl = l >> 1, e = l & 1;
+        const int bud = e ? p - 1 : p + 1;
+        const double2 b0 = pd[0][bud], b1 = pd[1][bud], b2 = pd[2][bud], b3 = pd[3][bud];
+        const double4 Ba = make_double4(b0.x, b0.y, b1.x, b1.y), Bb = make_double4(b2.x, b2.y, b3.x, b3.y);
+        double Rb = 0.0;
+        if (e == 0) {  // the couple's own triplet: the buddy is the partner
+          TriV v;
+          tri_locate(ux, uy, uz, Ba, v, span);
+          Rb = tri_apply(ux, uy, uz, fj, Aj, Vjx, Vjy, Vjz, Ba, Bb, v, Rs, span);
+        }
+        const bool odd = e == 1 && (n & 1);
+        const int trips = ((mc - 1) >> 1) + (((mc & 1) == 0 && cpl < (mc >> 1)) ? 1 : 0) + (odd ? 1 : 0);
+        int cn = odd ? cpl : (cpl + 1 == mc ? 0 : cpl + 1);
+        int kq = odd ? kb + n - 1 : kb + 2 * cn + e;
+        for (int m = 0; m < trips; ++m) {
+          const double2 q0 = pd[0][kq], q1 = pd[1][kq], q2 = pd[2][kq], q3 = pd[3][kq];
+          const double4 Pa = make_double4(q0.x, q0.y, q1.x, q1.y), Pb = make_double4(q2.x, q2.y, q3.x, q3.y);
+          // both triplets' g coefficients are read before either's atomics
+          TriV v0, v1;
+          tri_locate(ux, uy, uz, Pa, v0, span);
+          tri_locate(Ba.x, Ba.y, Ba.z, Pa, v1, span);
+          const double sh = tri_apply(ux, uy, uz, fj, Aj, Vjx, Vjy, Vjz, Pa, Pb, v0, Rs, span) +
+                            tri_apply(Ba.x, Ba.y, Ba.z, Ba.w, Bb.x, Bb.y, Bb.z, Bb.w, Pa, Pb, v1, Rb, span);
+          atomicAdd(&push[0][kq], sh);
+          cn = (cn + 1 == mc) ? 0 : cn + 1;
+          kq = kb + 2 * cn + e;
+        }
+        atomicAdd(&push[0][bud], Rb);
+      }
+      Phi = Rs;
+    };
+    if (!POINT && A.pot.g_span)
+      sweep(std::true_type{});
+    else
+      sweep(std::false_type{});
+    } else {
+    const int trips = ((n - 1) >> 1) + (((n & 1) == 0 && l < (n >> 1)) ? 1 : 0);
     double Rj = 0.0;
     GSpl Gq[NJ];  // q (q2) of the atom's type
 #pragma unroll
@@ -664,6 +833,7 @@ __global__ void __launch_bounds__(BLK) k_adjoint(const AdjArgs A) {
       fetch_partner(kb + kl, cur);  // the next visit's partner, read behind this visit's atomics
     }
     Phi = Rj;
+    }  // kCouples
   }
   // ---- phase D: pair functions.  Pairs of similar length hit the same
   // interval, so their coefficient adjoints go to RR lane replicas of the
@@ -676,7 +846,7 @@ __global__ void __launch_bounds__(BLK) k_adjoint(const AdjArgs A) {
   for (int q = tid; q < RS * RR; q += BLK) rrep[q] = 0.0;
   if (MEAM && has) {
     Phi += push[0][p];
-    Phip = Up[ia] * aj * S1;
+    if (POINT) Phip = Up[ia] * aj * S1;
 #pragma unroll
     for (int m = 0; m < JM; ++m) Rq[m] += push[2 + m][p];
   }
@@ -693,9 +863,13 @@ __global__ void __launch_bounds__(BLK) k_adjoint(const AdjArgs A) {
         acc_radial<POINT, Q>(atab, sk, ncoef, sf, HD[sf], IV, r, Phi, Phip);
       }
     } else {
-      if (phrec >= 0) acc_interval<Q>(sk, phrec, tphi, 0.5 * lamE, 0.5 * aj * phinv);
-      if (rrec >= 0) acc_interval<Q>(sk, rrec, trho, ps, aj * up * rinv);
-      if (MEAM && frec >= 0) acc_interval<Q>(sk, frec, tf, Phi, Phip * finv);
+      if (!MEAM) {
+        wphi = 0.5 * aj * phinv;
+        wrho = aj * up * rinv;
+      }
+      if (phrec >= 0) acc_interval<Q>(sk, phrec, tphi, 0.5 * lamE, wphi);
+      if (rrec >= 0) acc_interval<Q>(sk, rrec, trho, ps, wrho);
+      if (MEAM && frec >= 0) acc_interval<Q>(sk, frec, tf, Phi, wf);
     }
     // jmeam p splines: value adjoint Rq, p' adjoint a T1 (no U' factor)
 #pragma unroll
@@ -1076,6 +1250,7 @@ int setup_adjoint(mz_ctx* c) {
     for (int q = c->atom_poff[a]; q < c->atom_poff[a + 1]; ++q) owner[q] = a;
   if (X.d_owner.upload(owner.data(), owner.size())) return mz_fail(MZ_ERR_NOMEM, "adjoint owner upload failed");
   X.nchunks = (int)sorted.size();
+
   // Lane replicas of the radial interval records reuse the partner-data area.
   X.nrad = c->meam ? X.gco0 : c->ncoef;
   X.rrep_stride = ((X.nrad * nc_g + 31) & ~31) + 1;

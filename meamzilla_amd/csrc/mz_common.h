@@ -79,6 +79,8 @@ struct PotDesc {
   int32_t ng;           // number of g splines (nt, or nt * nt (nt + 1) / 2 with gpair)
   int32_t nodens;       // pair styles: phi only, no density, embedding or density penalty
   int32_t njm;          // jmeam styles: extra p_ij p_ik q(cos) energy terms (1, or 2 for jmeam2)
+  int32_t g_span;       // knot style, every g spline spans [-1, 1] (x_0 <= -1, no cut): u = (cos - x_0) / h
+                        // exceeds -1, so the interval is min(trunc(u), n - 2)
 };
 
 // Doubles per g-interval record in the compact LDS copies of the g tables

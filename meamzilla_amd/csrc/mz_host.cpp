@@ -133,6 +133,7 @@ int build_layout(mz_ctx* c, const mz_layout* L) {
   // |cos| <= 1 + a few ulps, so a knot-style g spline ending at or beyond 1
   // never cuts a triplet
   c->pot.g_keep_all = 0;
+  c->pot.g_span = 0;
   if (c->meam && !c->point) {
     bool all = true;
     for (int t = 0; t < counts[4]; ++t) {
@@ -140,6 +141,10 @@ int build_layout(mz_ctx* c, const mz_layout* L) {
       if (!(c->xn[s] * (1. + 1.e-8) > 1.0 + 1.e-12)) all = false;
     }
     c->pot.g_keep_all = all ? 1 : 0;
+    bool span = all;
+    for (int t = 0; t < counts[4]; ++t)
+      if (!(c->x0[c->pot.fn_off[4] + t] <= -1.0)) span = false;
+    c->pot.g_span = span ? 1 : 0;
   }
   return MZ_OK;
 }
