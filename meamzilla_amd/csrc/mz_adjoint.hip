@@ -506,11 +506,6 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu((MEAM 
   for (int m = 0; m < NJ; ++m) Rq[m] = 0.0;
   if (MEAM && has) {
     const double ps = psi[ia], up = Up[ia];
-    if (!POINT) {
-      wphi = 0.5 * aj * phinv;
-      wrho = aj * up * rinv;
-      wf = up * aj * S1 * finv;
-    }
     const int kb = poff[ia], ke = poff[ia + 1];
     const int n = ke - kb, l = p - kb;
     const int gfix = GP ? ti * (nt * (nt + 1) / 2) : ti;  // g of the atom's type (and first pair with GP)
@@ -867,9 +862,20 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu((MEAM 
         wphi = 0.5 * aj * phinv;
         wrho = aj * up * rinv;
       }
-      if (phrec >= 0) acc_interval<Q>(sk, phrec, tphi, 0.5 * lamE, wphi);
-      if (rrec >= 0) acc_interval<Q>(sk, rrec, trho, ps, wrho);
-      if (MEAM && frec >= 0) acc_interval<Q>(sk, frec, tf, Phi, wf);
+      if (MEAM) {
+        // located again from r (knot positions in LDS): fewer values live across the sweep
+        int k;
+        double t;
+        const SplHead& Hp = HD[sphi];
+        if (locate_knot(Hp, kg + Hp.coef_off, r, k, t)) acc_interval<Q>(sk, Hp.coef_off + k, t, 0.5 * lamE, 0.5 * aj * Hp.invstep);
+        const SplHead& Hr = HD[srho];
+        if (locate_knot(Hr, kg + Hr.coef_off, r, k, t)) acc_interval<Q>(sk, Hr.coef_off + k, t, ps, aj * up * Hr.invstep);
+        const SplHead& Hf = HD[A.pot.fn_off[3] + pij];
+        if (locate_knot(Hf, kg + Hf.coef_off, r, k, t)) acc_interval<Q>(sk, Hf.coef_off + k, t, Phi, up * aj * S1 * Hf.invstep);
+      } else {
+        if (phrec >= 0) acc_interval<Q>(sk, phrec, tphi, 0.5 * lamE, wphi);
+        if (rrec >= 0) acc_interval<Q>(sk, rrec, trho, ps, wrho);
+      }
     }
     // jmeam p splines: value adjoint Rq, p' adjoint a T1 (no U' factor)
 #pragma unroll

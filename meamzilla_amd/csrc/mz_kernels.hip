@@ -23,6 +23,8 @@
 // sum over the pairs that point at it, gathered in k_resid in a fixed order.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "mz_common.h"
 #include "mz_kernels.h"
 #include "mz_device.h"
@@ -290,18 +292,10 @@ __device__ __forceinline__ void g_value(const SplHead& Hg, const double2* gtg, i
     gp = g1 + g2 * e;
     return;
   }
-  const double t = L.t;
   const double2* C = gtg + L.off;
-  const double2 c01 = C[0], c23 = C[ps];
+  const double2 c01 = C[0], c23 = C[ps], c45 = Q ? C[2 * ps] : c23;
   double dgt;
-  if (Q) {
-    const double2 c45 = C[2 * ps];
-    gv = c01.x + t * (c01.y + t * (c23.x + t * (c23.y + t * (c45.x + t * c45.y))));
-    dgt = c01.y + t * (2.0 * c23.x + t * (3.0 * c23.y + t * (4.0 * c45.x + t * 5.0 * c45.y)));
-  } else {
-    gv = c01.x + t * (c01.y + t * (c23.x + t * c23.y));
-    dgt = c01.y + t * (2.0 * c23.x + t * 3.0 * c23.y);
-  }
+  horner_d<Q>(c01, c23, c45, L.t, gv, dgt);
   gp = dgt * Hg.invstep;
 }
 
@@ -527,14 +521,29 @@ __global__ void __launch_bounds__(BLOCK) k_atoms(const AtomArgs args) {
 #pragma unroll
           for (int m = 0; m < NJ; ++m) T[m][c][q] = 0.0;
         }
+      const double gx0 = Hg0.x0, ginv = Hg0.invstep;
+      const int gkm = Hg0.n - 2;
+      auto walk = [&](auto spn) {
       for (int k = kb; k < ke; ++k) {
         const double4 P = sm.rec[k];
         const double cs = xj * P.x + yj * P.y + zj * P.z;
         const int gi = GP ? gfix + pair_ij(tjs[s], sm.ptype[k], nt) : gfix;
         const SplHead& Hg = GP ? GHc(0)[gi] : Hg0;
         const int goff = GP ? Hg.coef_off - args.pot.gco0 : goff0;
-        const GLoc L = g_locate<POINT, Q>(Hg, cs);
-        const bool zero = k == p || (cut && !(cs < Hg.xcut));
+        GLoc L;
+        bool zero;
+        if constexpr (decltype(spn)::value) {
+          // the spline spans [-1, 1]: u > -1, truncation gives the clamped floor
+          const double u = (cs - gx0) * ginv;
+          const int kk = (int)u;
+          L.off = kk < gkm ? kk : gkm;
+          L.t = u - (double)L.off;
+          L.side = 0;
+          zero = k == p;
+        } else {
+          L = g_locate<POINT, Q>(Hg, cs);
+          zero = k == p || (cut && !(cs < Hg.xcut));
+        }
 #pragma unroll
         for (int c = 0; c < CT; ++c) {
           double gv, gp;
@@ -570,6 +579,11 @@ __global__ void __launch_bounds__(BLOCK) k_atoms(const AtomArgs args) {
           }
         }
       }
+      };
+      if (!POINT && !GP && args.pot.g_span)
+        walk(std::true_type{});
+      else
+        walk(std::false_type{});
 #pragma unroll
       for (int c = 0; c < CT; ++c) {
         sm.rp[c][p] += 0.5 * fv[s][c] * S[c][0];
