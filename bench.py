@@ -293,7 +293,8 @@ def measure(wl: int, steps: int, warmup: int, mode: str, cand: int, side: bool, 
         # the work decomposition the library's autotuner picks for this
         # database (once per database, like the list build; not timed)
         t0 = time.time()
-        obj.autotune(ncand=min(cand, 32), reps=2)
+        nt = min(cand, 32)
+        obj.autotune(ncand=nt, reps=2, grad=mode in ("batch", "adjoint"), knots=w.candidate_knots(nt))
         tune_s = time.time() - t0
         t = obj.get_tile()
         log(f"autotuned tile {t.max_atoms},{t.max_pairs},{t.block},{t.cand_tile} in {tune_s:.1f}s")

@@ -174,9 +174,15 @@ int mz_residuals_batch(mz_ctx* ctx, int ncand, const double* knots, double* F, d
 
 /* Work-decomposition tile: atoms per chunk limit, pairs per chunk limit and
  * candidates per block.  mz_autotune times tiles on the current database with
- * ncand knot vectors per call (a coordinate search: candidates per workgroup,
- * then threads and pairs per chunk, then atoms per chunk) and keeps the
- * fastest; mz_get_tile reports the tile in use. */
+ * ncand all-zero knot vectors per call (a coordinate search: candidates per
+ * workgroup, then threads and pairs per chunk, then atoms per chunk; the
+ * fastest of reps calls per tile) and keeps the fastest.  mz_autotune_knots
+ * times with the caller's ncand knot vectors knots[ncand*nknot] and, when grad
+ * is nonzero, with objective-and-gradient calls: the time depends a little on
+ * the values (stores of all-zero data are cheaper), and gradient calls also
+ * store the leg sums the adjoint reads, so representative vectors in the mode
+ * the caller will use rank the tiles as the real calls do.  mz_get_tile
+ * reports the tile in use.  No reference counterpart (work decomposition). */
 typedef struct {
   int max_atoms;        /* atoms per chunk (work unit) */
   int max_pairs;        /* pairs per chunk */
@@ -186,6 +192,7 @@ typedef struct {
 int mz_set_tile(mz_ctx* ctx, const mz_tile* tile);
 int mz_get_tile(const mz_ctx* ctx, mz_tile* tile);
 int mz_autotune(mz_ctx* ctx, int ncand, int reps, mz_tile* best);
+int mz_autotune_knots(mz_ctx* ctx, int ncand, const double* knots, int grad, int reps, mz_tile* best);
 
 /* Device timing of the dominant kernels over the last call made with timing
  * enabled: HIP events recorded on the context stream around each kernel

@@ -87,6 +87,7 @@ def lib() -> C.CDLL:
     L.mz_set_tile.argtypes = [vp, C.POINTER(Tile)]
     L.mz_get_tile.argtypes = [vp, C.POINTER(Tile)]
     L.mz_autotune.argtypes = [vp, C.c_int, C.c_int, C.POINTER(Tile)]
+    L.mz_autotune_knots.argtypes = [vp, C.c_int, dp, C.c_int, C.c_int, C.POINTER(Tile)]
     L.mz_enable_timing.argtypes = [vp, C.c_int]
     L.mz_last_timing.argtypes = [vp, C.POINTER(Timing)]
     L.mz_spline_refresh.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int, dp, dp, dp]
@@ -246,9 +247,20 @@ class Objective:
         _check(lib().mz_get_tile(self._ctx, C.byref(t)))
         return t
 
-    def autotune(self, ncand: int = 1, reps: int = 3) -> Tile:
+    def autotune(self, ncand: int = 1, reps: int = 3, grad: bool = False,
+                 knots: Optional[np.ndarray] = None) -> Tile:
+        """Times tiles with batched calls and keeps the fastest: objective
+        calls on zero vectors by default; with knots (ncand vectors) and grad,
+        the caller's vectors in objective-and-gradient calls."""
         t = Tile()
-        _check(lib().mz_autotune(self._ctx, int(ncand), int(reps), C.byref(t)))
+        if knots is None and not grad:
+            _check(lib().mz_autotune(self._ctx, int(ncand), int(reps), C.byref(t)))
+        else:
+            kv = None if knots is None else self._kv(np.atleast_2d(knots))
+            if kv is not None and kv.shape[0] != ncand:
+                raise ValueError(f"{kv.shape[0]} knot vectors for ncand={ncand}")
+            _check(lib().mz_autotune_knots(self._ctx, int(ncand), _dp(kv), 1 if grad else 0, int(reps),
+                                           C.byref(t)))
         _check(lib().mz_info(self._ctx, C.byref(self.info)))
         return t
 

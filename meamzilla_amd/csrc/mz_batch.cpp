@@ -26,13 +26,21 @@ int mz_residuals_batch(mz_ctx* c, int ncand, const double* knots, double* F, dou
   return mz_residuals_batch_impl(c, ncand, knots, F, errvec);
 }
 
+}  // extern "C"
+
+namespace {
+
 // Coordinate search over the tile: candidates per workgroup, then the
 // (threads, pairs per chunk) shape, then atoms per chunk, each dimension timed
 // with the others held at the best so far (about 12 timings instead of the 54
-// combinations).  Each timing is one warm-up and `reps` calls of
-// mz_objective_batch with ncand vectors; the work per call does not depend on
-// the knot values, so any finite vector times it.
-int mz_autotune(mz_ctx* c, int ncand, int reps, mz_tile* best) {
+// combinations).  Each timing is one warm-up call and the fastest of `reps`
+// calls of mz_objective_batch with the ncand knot vectors `knots` (zeros when
+// null), objective only or, with grad, objective and gradient (the forward
+// kernel then also stores the leg sums the adjoint reads).  The arithmetic
+// does not depend on the knot values but the time does, a little: stores of
+// all-zero data are cheaper, which on the ternary workload ranks the tiles
+// differently, so callers that have representative vectors pass them.
+int autotune(mz_ctx* c, int ncand, const double* knots, bool grad, int reps, mz_tile* best) {
   if (!c || ncand < 1) return MZ_ERR_ARG;
   if (reps < 1) reps = 3;
   mz_info_t info;
@@ -40,13 +48,20 @@ int mz_autotune(mz_ctx* c, int ncand, int reps, mz_tile* best) {
   mz_tile bt;
   mz_get_tile(c, &bt);
   std::vector<double> kv((size_t)ncand * info.nknot, 0.0);
+  if (knots) std::memcpy(kv.data(), knots, kv.size() * sizeof(double));
   std::vector<double> Fv(ncand);
+  std::vector<double> Gv(grad ? (size_t)ncand * info.nknot : 0);
+  double* G = grad ? Gv.data() : nullptr;
   auto time_tile = [&](const mz_tile& t, double& dt) {
     if (mz_set_tile(c, &t) != MZ_OK) return false;
-    if (mz_objective_batch(c, ncand, kv.data(), Fv.data(), nullptr) != MZ_OK) return false;
-    auto t0 = std::chrono::steady_clock::now();
-    for (int r = 0; r < reps; ++r) mz_objective_batch(c, ncand, kv.data(), Fv.data(), nullptr);
-    dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() / reps;
+    if (mz_objective_batch(c, ncand, kv.data(), Fv.data(), G) != MZ_OK) return false;
+    dt = 1e300;
+    for (int r = 0; r < reps; ++r) {
+      auto t0 = std::chrono::steady_clock::now();
+      if (mz_objective_batch(c, ncand, kv.data(), Fv.data(), G) != MZ_OK) return false;
+      const double d = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+      if (d < dt) dt = d;
+    }
     return true;
   };
   double best_t = 1e300;
@@ -86,6 +101,18 @@ int mz_autotune(mz_ctx* c, int ncand, int reps, mz_tile* best) {
   mz_set_tile(c, &bt);
   if (best) *best = bt;
   return MZ_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mz_autotune(mz_ctx* c, int ncand, int reps, mz_tile* best) {
+  return autotune(c, ncand, nullptr, false, reps, best);
+}
+
+int mz_autotune_knots(mz_ctx* c, int ncand, const double* knots, int grad, int reps, mz_tile* best) {
+  return autotune(c, ncand, knots, grad != 0, reps, best);
 }
 
 }  // extern "C"

@@ -219,6 +219,27 @@ def test_gpu_autotune_keeps_results():
     assert np.max(np.abs(F1 - F0) / np.abs(F0)) < 1e-12
 
 
+def test_gpu_autotune_grad_keeps_results():
+    """The autotuner timing the caller's vectors in gradient calls picks a
+    supported tile and the batched
+    objective and gradient of every candidate are unchanged under it (to the
+    round-off of the adjoint's LDS accumulation order)."""
+    _say("autotune grad")
+    w = gen.workload(1, scale=8 / 256)
+    gen.add_targets(w, pyoracle.evaluator)
+    obj = Objective(w.pot, w.db)
+    K = w.candidate_knots(4)
+    F0, G0 = obj.batch(K, grad=True)
+    t = obj.autotune(ncand=4, reps=1, grad=True, knots=K)
+    assert t.cand_tile in (1, 2, 4) and t.block in (64, 128, 256, 512)
+    g = obj.get_tile()
+    assert (g.max_atoms, g.max_pairs, g.block, g.cand_tile) == (t.max_atoms, t.max_pairs, t.block, t.cand_tile)
+    F1, G1 = obj.batch(K, grad=True)
+    assert np.max(np.abs(F1 - F0) / np.abs(F0)) < 1e-12
+    for q in range(4):
+        assert rel_err(G1[q], G0[q]) < 1e-12, q
+
+
 def test_gpu_residuals_batch_matches_plain():
     """Batched residual vectors (Powell's Jacobian columns) equal the plain
     evaluation's residual vector for every candidate."""
