@@ -284,6 +284,12 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu((MEAM 
   // once per workgroup, and each candidate's partial tables are summed by
   // k_chain in a fixed order (no global atomics).
   for (int ci = blockIdx.x; ci < A.nchunks; ci += gridDim.x) {
+  // the lane index, opaque to the compiler inside each chunk: addresses
+  // derived from it are recomputed per chunk instead of held (and spilled)
+  // across the whole loop
+  int tidc = threadIdx.x;
+  asm volatile("" : "+v"(tidc));
+  const int tid = tidc;
   const Chunk ch = A.chunks[ci];
   const int p = tid;
   const bool has = p < ch.np;
