@@ -504,13 +504,13 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu((MEAM 
   // when n is even (the forward pass's round-robin); the partner sums go
   // through LDS atomics on distinct k.
   double Phi = 0.0, Phip = 0.0;
-  // knot style, meam: phase D's slope weights of the phi, rho and f
-  // intervals, formed before the sweep (fewer registers live across it)
-  double wphi = 0.0, wrho = 0.0, wf = 0.0;
   double Rq[NJ];  // jmeam: p-value adjoint of the lane's pair per term, own visits
 #pragma unroll
   for (int m = 0; m < NJ; ++m) Rq[m] = 0.0;
   if (MEAM && has) {
+    // knot style: the rho and f intervals located in phase A wait in the
+    // staging slot (free from phase B to the next chunk) for phase D
+    if (!POINT) reinterpret_cast<int2*>(&push[1][0])[p] = make_int2(rrec, frec);
     const double ps = psi[ia], up = Up[ia];
     const int kb = poff[ia], ke = poff[ia + 1];
     const int n = ke - kb, l = p - kb;
@@ -864,23 +864,26 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu((MEAM 
         acc_radial<POINT, Q>(atab, sk, ncoef, sf, HD[sf], IV, r, Phi, Phip);
       }
     } else {
-      if (!MEAM) {
-        wphi = 0.5 * aj * phinv;
-        wrho = aj * up * rinv;
-      }
       if (MEAM) {
-        // located again from r (knot positions in LDS): fewer values live across the sweep
+        // phi located here, rho and f at the intervals phase A found, their
+        // shifts recomputed from r (fewer values live across the sweep)
         int k;
         double t;
         const SplHead& Hp = HD[sphi];
-        if (locate_knot(Hp, kg + Hp.coef_off, r, k, t)) acc_interval<Q>(sk, Hp.coef_off + k, t, 0.5 * lamE, 0.5 * aj * Hp.invstep);
-        const SplHead& Hr = HD[srho];
-        if (locate_knot(Hr, kg + Hr.coef_off, r, k, t)) acc_interval<Q>(sk, Hr.coef_off + k, t, ps, aj * up * Hr.invstep);
-        const SplHead& Hf = HD[A.pot.fn_off[3] + pij];
-        if (locate_knot(Hf, kg + Hf.coef_off, r, k, t)) acc_interval<Q>(sk, Hf.coef_off + k, t, Phi, up * aj * S1 * Hf.invstep);
+        if (locate_knot(Hp, kg + Hp.coef_off, r, k, t))
+          acc_interval<Q>(sk, Hp.coef_off + k, t, 0.5 * lamE, 0.5 * aj * Hp.invstep);
+        const int2 rf = reinterpret_cast<const int2*>(&push[1][0])[p];
+        if (rf.x >= 0) {
+          const KnotRec K = kg[rf.x];
+          acc_interval<Q>(sk, rf.x, (r - K.xk) * K.invh, ps, aj * up * HD[srho].invstep);
+        }
+        if (rf.y >= 0) {
+          const KnotRec K = kg[rf.y];
+          acc_interval<Q>(sk, rf.y, (r - K.xk) * K.invh, Phi, up * aj * S1 * HD[A.pot.fn_off[3] + pij].invstep);
+        }
       } else {
-        if (phrec >= 0) acc_interval<Q>(sk, phrec, tphi, 0.5 * lamE, wphi);
-        if (rrec >= 0) acc_interval<Q>(sk, rrec, trho, ps, wrho);
+        if (phrec >= 0) acc_interval<Q>(sk, phrec, tphi, 0.5 * lamE, 0.5 * aj * phinv);
+        if (rrec >= 0) acc_interval<Q>(sk, rrec, trho, ps, aj * up * rinv);
       }
     }
     // jmeam p splines: value adjoint Rq, p' adjoint a T1 (no U' factor)
