@@ -508,9 +508,12 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu((MEAM 
 #pragma unroll
   for (int m = 0; m < NJ; ++m) Rq[m] = 0.0;
   if (MEAM && has) {
-    // knot style: the rho and f intervals located in phase A wait in the
-    // staging slot (free from phase B to the next chunk) for phase D
-    if (!POINT) reinterpret_cast<int2*>(&push[1][0])[p] = make_int2(rrec, frec);
+    // knot style: the rho, f and phi intervals located in phase A wait in the
+    // staging slot (free from phase B to the next chunk) for phase D, three
+    // 16-bit record indices (0xffff: none; the host keeps ncoef below it)
+    if (!POINT)
+      reinterpret_cast<int2*>(&push[1][0])[p] =
+          make_int2((int)(((unsigned)rrec & 0xffffu) | ((unsigned)frec << 16)), phrec);
     const double ps = psi[ia], up = Up[ia];
     const int kb = poff[ia], ke = poff[ia + 1];
     const int n = ke - kb, l = p - kb;
@@ -865,21 +868,21 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu((MEAM 
       }
     } else {
       if (MEAM) {
-        // phi located here, rho and f at the intervals phase A found, their
-        // shifts recomputed from r (fewer values live across the sweep)
-        int k;
-        double t;
-        const SplHead& Hp = HD[sphi];
-        if (locate_knot(Hp, kg + Hp.coef_off, r, k, t))
-          acc_interval<Q>(sk, Hp.coef_off + k, t, 0.5 * lamE, 0.5 * aj * Hp.invstep);
+        // phi, rho and f at the intervals phase A located, their shifts
+        // recomputed from r (fewer values live across the sweep)
         const int2 rf = reinterpret_cast<const int2*>(&push[1][0])[p];
-        if (rf.x >= 0) {
-          const KnotRec K = kg[rf.x];
-          acc_interval<Q>(sk, rf.x, (r - K.xk) * K.invh, ps, aj * up * HD[srho].invstep);
+        const int rr = (int)((unsigned)rf.x & 0xffffu), fr = (int)((unsigned)rf.x >> 16), pr = rf.y;
+        if (pr >= 0) {
+          const KnotRec K = kg[pr];
+          acc_interval<Q>(sk, pr, (r - K.xk) * K.invh, 0.5 * lamE, 0.5 * aj * HD[sphi].invstep);
         }
-        if (rf.y >= 0) {
-          const KnotRec K = kg[rf.y];
-          acc_interval<Q>(sk, rf.y, (r - K.xk) * K.invh, Phi, up * aj * S1 * HD[A.pot.fn_off[3] + pij].invstep);
+        if (rr != 0xffff) {
+          const KnotRec K = kg[rr];
+          acc_interval<Q>(sk, rr, (r - K.xk) * K.invh, ps, aj * up * HD[srho].invstep);
+        }
+        if (fr != 0xffff) {
+          const KnotRec K = kg[fr];
+          acc_interval<Q>(sk, fr, (r - K.xk) * K.invh, Phi, up * aj * S1 * HD[A.pot.fn_off[3] + pij].invstep);
         }
       } else {
         if (phrec >= 0) acc_interval<Q>(sk, phrec, tphi, 0.5 * lamE, 0.5 * aj * phinv);
@@ -1184,6 +1187,8 @@ int setup_adjoint(mz_ctx* c) {
   // candidates per launch: 512 x 16 (16 waves) 402 ms, 256 x 8 (12 waves)
   // 515 ms, 256 x 1 (16 waves) and 512 x 32 (8 waves) 920 ms; one-element
   // 256-cell workload: 256 x 32 about 8 % ahead of 512 x 32 (both 16 waves).
+  // phase D finds the located interval records as 16-bit indices
+  if (c->ncoef >= 0xffff) return mz_fail(MZ_ERR_CAPACITY, "more than 65534 spline intervals");
   X.ntab = c->ncoef * 6 + c->nspl * 6;
   X.gco0 = c->meam ? c->coef_off[c->pot.fn_off[4]] : 0;
   X.ngco = c->meam ? c->ncoef - X.gco0 : 0;
