@@ -321,8 +321,12 @@ struct AtomSmem {
 // JM: jmeam styles, JM extra energy terms p_ij p_ik q(cos) per triplet
 // (src/pots/pot_jmeam/pot_jmeam.cpp:221,263-265; jmeam2 adds p2 p2 q2,
 // src/pots/pot_jmeam2/pot_jmeam2.cpp:240-241,284-286), walked beside g.
+// The 256-thread, one-pair-per-lane, four-candidate tile is held to three
+// waves per SIMD (168 VGPRs, no spill; 169 would leave two): its 28 KB of LDS
+// then fit three workgroups per CU.
 template <int BLOCK, int PPL, int MAXA, int CT, bool MEAM, bool POINT, bool Q, bool TG, bool GP, int JM>
-__global__ void __launch_bounds__(BLOCK) k_atoms(const AtomArgs args) {
+__global__ void __launch_bounds__(BLOCK)
+    __attribute__((amdgpu_waves_per_eu((CT == 4 && BLOCK == 256 && PPL == 1) ? 3 : 1))) k_atoms(const AtomArgs args) {
   extern __shared__ __attribute__((aligned(64))) char dyn[];
   __shared__ __attribute__((aligned(16))) AtomSmem<BLOCK, PPL, MAXA, CT, GP, JM> sm;
   constexpr int NJ = JM > 0 ? JM : 1;
