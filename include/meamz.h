@@ -174,8 +174,8 @@ int mz_residuals_batch(mz_ctx* ctx, int ncand, const double* knots, double* F, d
 
 /* Work-decomposition tile: atoms per chunk limit, pairs per chunk limit and
  * candidates per block.  mz_autotune times tiles on the current database with
- * ncand all-zero knot vectors per call (a coordinate search: candidates per
- * workgroup, then threads and pairs per chunk, then atoms per chunk; the
+ * ncand all-zero knot vectors per call (every pair of candidates per
+ * workgroup and threads and pairs per chunk, then atoms per chunk; the
  * fastest of reps calls per tile) and keeps the fastest.  mz_autotune_knots
  * times with the caller's ncand knot vectors knots[ncand*nknot] and, when grad
  * is nonzero, with objective-and-gradient calls: the time depends a little on

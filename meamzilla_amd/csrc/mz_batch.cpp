@@ -30,10 +30,9 @@ int mz_residuals_batch(mz_ctx* c, int ncand, const double* knots, double* F, dou
 
 namespace {
 
-// Coordinate search over the tile: candidates per workgroup, then the
-// (threads, pairs per chunk) shape, then atoms per chunk, each dimension timed
-// with the others held at the best so far (about 12 timings instead of the 54
-// combinations).  Each timing is one warm-up call and the fastest of `reps`
+// Search over the tile: every (candidates per workgroup, threads and pairs
+// per chunk) pair, then atoms per chunk with the best of those (about 20
+// timings instead of the 54 combinations).  Each timing is one warm-up call and the fastest of `reps`
 // calls of mz_objective_batch with the ncand knot vectors `knots` (zeros when
 // null), objective only or, with grad, objective and gradient (the forward
 // kernel then also stores the leg sums the adjoint reads).  The arithmetic
@@ -76,21 +75,18 @@ int autotune(mz_ctx* c, int ncand, const double* knots, bool grad, int reps, mz_
   static const int ctiles[] = {1, 2, 4};
   static const int shapes[][2] = {{256, 512}, {256, 256}, {128, 256}, {128, 512}, {512, 512}, {64, 256}};
   static const int atoms[] = {16, 32, 64};
+  // candidates per workgroup and shape together: their best pairs differ
+  // (registers per lane grow with both), then atoms per chunk
   const mz_tile start = bt;
   for (int ct : ctiles)
-    if (ct != start.cand_tile && ct <= ncand) {
-      mz_tile t = bt;
-      t.cand_tile = ct;
-      consider(t);
-    }
-  const mz_tile after_ct = bt;
-  for (auto& sh : shapes)
-    if (sh[0] != after_ct.block || sh[1] != after_ct.max_pairs) {
-      mz_tile t = bt;
-      t.block = sh[0];
-      t.max_pairs = sh[1];
-      consider(t);
-    }
+    for (auto& sh : shapes)
+      if (ct <= ncand && (ct != start.cand_tile || sh[0] != start.block || sh[1] != start.max_pairs)) {
+        mz_tile t = start;
+        t.cand_tile = ct;
+        t.block = sh[0];
+        t.max_pairs = sh[1];
+        consider(t);
+      }
   const mz_tile after_shape = bt;
   for (int na : atoms)
     if (na != after_shape.max_atoms) {
