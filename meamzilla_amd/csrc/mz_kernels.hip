@@ -738,7 +738,9 @@ __global__ void __launch_bounds__(BLOCK)
 
 // ------------------------------------------------------------------ k_resid
 
-// Eight lanes per item; items are the atoms, then the cells.  An atom's lanes
+// Eight lanes per item; items are the cells, then the atoms (a cell's sum
+// over its chunks is a serial chain of loads, up to a few hundred long, which
+// then runs beside the atoms' streams instead of after them).  An atom's lanes
 // gather the force vectors of the pairs pointing at it (F_a = own sum minus
 // incoming sum, fixed order, no atomics) and form its three force residuals;
 // a cell's lanes each reduce one of the eight per-chunk accumulators (energy,
@@ -750,11 +752,11 @@ template <int BLOCK>
 __global__ void __launch_bounds__(BLOCK) k_resid(const ResidArgs a) {
   __shared__ double red[BLOCK / 64];
   const int cand = blockIdx.y;
-  const int item = (blockIdx.x * BLOCK + threadIdx.x) / kResidLanes;
+  const int item = (blockIdx.x * BLOCK + threadIdx.x) / kResidLanes - a.ncells;  // cells: -ncells .. -1
   const int l = threadIdx.x & (kResidLanes - 1);
   double sq = 0.0;
   double* ev = a.errvec ? a.errvec + (size_t)cand * a.nerr : nullptr;
-  if (item < a.natoms) {
+  if (item >= 0 && item < a.natoms) {
     const int i = item;
     const double* tv = a.tvec + (size_t)cand * a.npairs * 3;
     double sx = 0.0, sy = 0.0, sz = 0.0;
@@ -780,8 +782,8 @@ __global__ void __launch_bounds__(BLOCK) k_resid(const ResidArgs a) {
       if (ev) ev[7 * (size_t)(c + 1) + 3 * (size_t)i + l] = er;
       if (a.lam) a.lam[ai] = 2.0 * er * w;
     }
-  } else if (item < a.natoms + a.ncells) {
-    const int c = item - a.natoms;
+  } else if (item < 0) {
+    const int c = item + a.ncells;
     const double* cacc = a.chunk_acc + (size_t)cand * a.nchunks * 8;
     double acc = 0.0;
     for (int q = a.cell_chunk_off[c]; q < a.cell_chunk_off[c + 1]; ++q) acc += cacc[8 * (size_t)a.cell_chunks[q] + l];
