@@ -822,8 +822,18 @@ __global__ void __launch_bounds__(BLOCK) k_resid(const ResidArgs a) {
 __global__ void __launch_bounds__(256) k_final(const FinalArgs a) {
   __shared__ double red[4];
   const int cand = blockIdx.x;
-  double s = 0.0;
-  for (int q = threadIdx.x; q < a.npartial; q += 256) s += a.partial[(size_t)cand * a.npartial + q];
+  // eight independent partial sums per thread keep eight loads in flight
+  // (a single chain is bound by load latency on the ~72k ternary partials),
+  // combined in a fixed order
+  const double* pa = a.partial + (size_t)cand * a.npartial;
+  double s8[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  int q = threadIdx.x;
+  for (; q + 7 * 256 < a.npartial; q += 8 * 256) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) s8[u] += pa[q + u * 256];
+  }
+  for (; q < a.npartial; q += 256) s8[0] += pa[q];
+  const double s = ((s8[0] + s8[1]) + (s8[2] + s8[3])) + ((s8[4] + s8[5]) + (s8[6] + s8[7]));
   const double tot = block_sum<256>(s, red);
   if (threadIdx.x != 0) return;
   double F = tot;
