@@ -785,8 +785,17 @@ __global__ void __launch_bounds__(BLOCK) k_resid(const ResidArgs a) {
   } else if (item < 0) {
     const int c = item + a.ncells;
     const double* cacc = a.chunk_acc + (size_t)cand * a.nchunks * 8;
-    double acc = 0.0;
-    for (int q = a.cell_chunk_off[c]; q < a.cell_chunk_off[c + 1]; ++q) acc += cacc[8 * (size_t)a.cell_chunks[q] + l];
+    // four independent partial sums (four index-then-value load pairs in
+    // flight), combined in a fixed order
+    double a4[4] = {0.0, 0.0, 0.0, 0.0};
+    int q = a.cell_chunk_off[c];
+    const int qe = a.cell_chunk_off[c + 1];
+    for (; q + 3 < qe; q += 4) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) a4[u] += cacc[8 * (size_t)a.cell_chunks[q + u] + l];
+    }
+    for (; q < qe; ++q) a4[0] += cacc[8 * (size_t)a.cell_chunks[q] + l];
+    const double acc = (a4[0] + a4[1]) + (a4[2] + a4[3]);
     const int n = a.natoms_c[c];
     const size_t e0 = 7 * (size_t)c + 3 * (size_t)a.abase[c];
     double* adj = a.cell_adj ? a.cell_adj + ((size_t)cand * a.ncells + c) * 8 : nullptr;
