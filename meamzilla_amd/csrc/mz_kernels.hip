@@ -761,7 +761,27 @@ __global__ void __launch_bounds__(BLOCK) k_resid(const ResidArgs a) {
     const double* tv = a.tvec + (size_t)cand * a.npairs * 3;
     double sx = 0.0, sy = 0.0, sz = 0.0;
     const int b = a.in_off[i], e = a.in_off[i + 1];
-    for (int q = b + l; q < e; q += kResidLanes) {
+    // three of the lane's pairs per iteration (nine loads in flight), each
+    // lane's sums still taken in ascending pair order
+    constexpr int U = 3;
+    int q = b + l;
+    for (; q + kResidLanes * (U - 1) < e; q += kResidLanes * U) {
+      double x[U], y[U], z[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const double* t = tv + 3 * (size_t)(q + kResidLanes * u);
+        x[u] = t[0];
+        y[u] = t[1];
+        z[u] = t[2];
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        sx += x[u];
+        sy += y[u];
+        sz += z[u];
+      }
+    }
+    for (; q < e; q += kResidLanes) {
       const double* t = tv + 3 * (size_t)q;
       sx += t[0];
       sy += t[1];
