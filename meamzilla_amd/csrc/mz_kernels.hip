@@ -393,8 +393,10 @@ __global__ void __launch_bounds__(BLOCK)
   // (staging a shared copy of them in LDS per workgroup measured slower)
   const SplInterval* IV0 = IVc(0);
 
-  // per-lane pair state
-  int ia[PPL], tjs[PPL];
+  // per-lane pair state; ipos: the pair's slot in its neighbour's incoming
+  // run, read here with the pair record so that phase 4's scattered force
+  // store does not wait on it
+  int ia[PPL], tjs[PPL], ipos[PPL];
   double r[PPL], invr[PPL], ux[PPL], uy[PPL], uz[PPL];
   double aphi[PPL][CT], brho[PPL][CT], fv[PPL][CT], dfv[PPL][CT], Gx[PPL][CT], Gy[PPL][CT], Gz[PPL][CT],
       S1v[PPL][CT];
@@ -417,7 +419,9 @@ __global__ void __launch_bounds__(BLOCK)
 #pragma unroll
       for (int m = 0; m < NJ; ++m) pv[m][s][c] = dpv[m][s][c] = 0.0;
     }
+    ipos[s] = 0;
     if (p < ch.np) {
+      ipos[s] = args.in_pos[ch.p0 + p];
       int lo = 0, hi = ch.na - 1;
       while (lo < hi) {  // last atom with poff <= p
         const int mid = (lo + hi + 1) >> 1;
@@ -690,7 +694,7 @@ __global__ void __launch_bounds__(BLOCK)
           reinterpret_cast<double4*>(args.G_out)[pg] = make_double4(Gx[s][c], Gy[s][c], Gz[s][c], S1v[s][c]);
         // stored at the pair's slot in its neighbour's incoming run, which
         // k_resid then reads as one contiguous stream per atom
-        double* tv = args.tvec + ((size_t)cand[c] * args.npairs + args.in_pos[ch.p0 + p]) * 3;
+        double* tv = args.tvec + ((size_t)cand[c] * args.npairs + ipos[s]) * 3;
         tv[0] = tx;
         tv[1] = ty;
         tv[2] = tz;
